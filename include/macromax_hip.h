@@ -1,0 +1,94 @@
+/*
+ * C ABI of the MI355X (gfx950) Born-series library, libmacromax_hip.so.
+ *
+ * All array arguments are device pointers to contiguous complex arrays (interleaved re/im; complex64 when the
+ * plan's dtype is MMX_C64, complex128 for MMX_C128) laid out as MacroMax lays out its fields:
+ * [C, 1, n0, n1, n2] in C order, last axis fastest (MacroMax backend/__init__.py:130-144). `stream` is a
+ * hipStream_t (NULL = the null stream). Functions return 0 on success and a negative MMX_E* code on error;
+ * mmx_last_error() then describes it. No function throws across the ABI. A plan is used by one host thread.
+ */
+#ifndef MACROMAX_HIP_H
+#define MACROMAX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMX_C64 0
+#define MMX_C128 1
+
+#define MMX_OK 0
+#define MMX_EINVAL -1    /* bad argument (shape, size, kind) */
+#define MMX_EUNSUP -2    /* unsupported FFT length (prime factor > 13 or > 4096 points on one axis) */
+#define MMX_EHIP -3      /* HIP runtime error */
+#define MMX_ENOMEM -4
+
+typedef struct mmx_plan_s* mmx_plan_t;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mmx_version(void);
+/* Text of the last error on this thread (empty if none). */
+const char* mmx_last_error(void);
+
+/*
+ * Plan for a grid of `ndim` (1..3) axes with `shape[ndim]` samples and k0-scaled sample pitch `step[ndim]`.
+ * Holds twiddle tables and the k-vectors k_a = 2 pi fftfreq(n_a, step_a) (MacroMax utils/ft/grid.py:426-439)
+ * on the current device. Replaces the state that MacroMax BackEnd.__init__ builds
+ * (backend/__init__.py:43-67; pyFFTW plans in backend/numpy.py:46-89).
+ */
+int mmx_plan_create(mmx_plan_t* plan, int ndim, const int64_t* shape, const double* step, int dtype);
+int mmx_plan_destroy(mmx_plan_t plan);
+
+/*
+ * FFT over the spatial axes of `nbatch` consecutive fields of the plan's grid.
+ * direction -1: unnormalised forward transform; +1: inverse transform with 1/N.
+ * in may equal out. Replaces BackEnd.ft / BackEnd.ift (backend/__init__.py:234-257; torch.py:183-205;
+ * numpy.py:112-133).
+ */
+int mmx_fft(mmx_plan_t plan, const void* in, void* out, int64_t nbatch, int direction, void* stream);
+
+/*
+ * Per-problem preconditioner coefficients, an array of `nbatch` records of 6 reals of the plan's real type:
+ *   {alpha_re, alpha_im, a, c_re, c_im, 0}
+ * chi_eff = a chi + c I and s_eff = a s; alpha enters the Green's function. a = 1, c = 0 reproduce MacroMax's
+ * chi and s exactly; a rejected update (solver.py:1089-1094) changes alpha, a and c instead of chi and s.
+ */
+
+/*
+ * One Born iteration for `nbatch` problems on the plan's grid (MacroMax Solution.__iter__,
+ * solver.py:1070-1085):
+ *     d = chi_eff [ G(chi_eff E + s_eff) - E ],   Eout = E + d,
+ * G the scalar (ncomp == 1) or dyadic (ncomp == 3) Green's operator (solver.py:564-584,
+ * backend/__init__.py:606-649). chi_kind 1: chi is [1,1,grid] (isotropic); 2: chi is [3,3,grid] (needs ncomp 3).
+ * E, Eout: [nbatch][ncomp][grid]; s: [nbatch or 1][ncomp][grid] (s_bstride 0 shares one source);
+ * chi: [nbatch or 1][chi size] (chi_bstride 0 shares one medium). Eout is also the work buffer; E is only read.
+ * norms (device, 2*nbatch doubles): {sum |d|^2, sum |Eout|^2} per problem, accumulated in FP64 with a fixed
+ * reduction order. The caller decides acceptance (||d|| < ||d_prev||, solver.py:1083) and swaps E and Eout.
+ */
+int mmx_born_step(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* chi, int ncomp,
+                  int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                  double* norms, void* stream);
+
+/* out[0] = sum |x|^2 over n complex elements, FP64 accumulation (backend/__init__.py:872-874, torch.py:316-317). */
+int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stream);
+
+/*
+ * y = chi x per voxel for `nbatch` fields x of ncomp components (BackEnd.mul(chi, x), backend/__init__.py:415-444,
+ * torch.py:220-249, for chi of shape [1,1,grid] or [3,3,grid]). y may alias x.
+ */
+int mmx_chi_mul(mmx_plan_t plan, const void* chi, int chi_kind, const void* x, void* y, int ncomp,
+                int64_t nbatch, void* stream);
+
+/*
+ * out = Pi_L F = k (k . F) / k^2 with Pi_L(k=0) = I on the first ndim components and 0 on the others
+ * (BackEnd.longitudinal_projection_ft, backend/__init__.py:606-649). F, out: [3][grid]; out may alias F.
+ */
+int mmx_longitudinal_projection_ft(mmx_plan_t plan, const void* F, void* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MACROMAX_HIP_H */

@@ -1,0 +1,72 @@
+"""ctypes binding of libmacromax_hip.so (the C ABI declared in include/macromax_hip.h).
+
+The library is the only compute path of this package: if it is missing or cannot be loaded, every entry point
+raises. There is no CPU or pure-PyTorch substitute.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+__all__ = ['lib', 'LIB_PATH', 'check', 'NativeError', 'MMX_C64', 'MMX_C128']
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmacromax_hip.so')
+
+MMX_C64 = 0
+MMX_C128 = 1
+
+EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_fft', 'mmx_born_step',
+            'mmx_norm2', 'mmx_chi_mul', 'mmx_longitudinal_projection_ft')
+
+
+class NativeError(RuntimeError):
+    """An error reported by libmacromax_hip.so."""
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(L):
+    c_int, c_i64, c_p = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+    L.mmx_version.restype = c_int
+    L.mmx_version.argtypes = []
+    L.mmx_last_error.restype = ctypes.c_char_p
+    L.mmx_last_error.argtypes = []
+    L.mmx_plan_create.restype = c_int
+    L.mmx_plan_create.argtypes = [ctypes.POINTER(c_p), c_int, ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), c_int]
+    L.mmx_plan_destroy.restype = c_int
+    L.mmx_plan_destroy.argtypes = [c_p]
+    L.mmx_fft.restype = c_int
+    L.mmx_fft.argtypes = [c_p, c_p, c_p, c_i64, c_int, c_p]
+    L.mmx_born_step.restype = c_int
+    L.mmx_born_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
+    L.mmx_norm2.restype = c_int
+    L.mmx_norm2.argtypes = [c_p, c_p, c_i64, c_p, c_p]
+    L.mmx_chi_mul.restype = c_int
+    L.mmx_chi_mul.argtypes = [c_p, c_p, c_int, c_p, c_p, c_int, c_i64, c_p]
+    L.mmx_longitudinal_projection_ft.restype = c_int
+    L.mmx_longitudinal_projection_ft.argtypes = [c_p, c_p, c_p, c_p]
+    return L
+
+
+def lib():
+    """The loaded library (loaded once). Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise ImportError(f'{LIB_PATH} not found: build it with `python -m macromax_amd.build`.')
+                _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def check(rc: int, what: str = ''):
+    """Raise NativeError for a non-zero return code of the C ABI."""
+    if rc != 0:
+        msg = lib().mmx_last_error().decode(errors='replace')
+        if rc == -2:
+            raise ValueError(f'{what}: {msg}')
+        raise NativeError(f'{what} failed ({rc}): {msg}')

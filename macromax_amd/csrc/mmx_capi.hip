@@ -1,0 +1,554 @@
+// C ABI of libmacromax_hip.so: plans, pass scheduling and launches (declarations in include/macromax_hip.h).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/macromax_hip.h"
+#include "mmx_kernels.hpp"
+
+using namespace mmx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define MMX_HIP_CHECK(expr)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(MMX_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kMaxN = 4096;          // longest axis a single pass holds in LDS
+constexpr size_t kLdsBudget = 65536; // preferred dynamic LDS per workgroup (more only when one pencil needs it)
+constexpr int kElemsF = 8;           // complex elements per thread per Stockham pass (complex64)
+constexpr int kElemsD = 4;           // (complex128)
+// Larger per-thread budgets, used only when a tile of one long pencil would need more than 1024 threads.
+constexpr int kElemsFBig = 16;
+constexpr int kElemsDBig = 8;
+
+struct AxisPlan {
+    int N;
+    int nrad;
+    int rad[kMaxRadices];
+    int64_t inner;   // stride of the axis
+    int64_t nouter;  // blocks before the axis (per field)
+    void* tw;        // device twiddles (forward), N complex
+};
+
+}  // namespace
+
+struct mmx_plan_s {
+    int dtype;
+    int ndim;
+    int64_t shape3[3];
+    int64_t V;
+    int naxes;
+    AxisPlan ax[3];  // non-trivial axes, outermost first
+    void* kvec;      // device k-vectors, n0 + n1 + n2 reals
+    int k1ofs, k2ofs;
+    void* scratch;   // FP64 partial sums
+    size_t scratch_bytes;
+    int device;
+    size_t lds_max;  // device limit of shared memory per block
+};
+
+namespace {
+
+bool factorize(int n, int* rad, int& nrad) {
+    static const int pref[] = {8, 4, 2, 3, 5, 7};
+    nrad = 0;
+    int m = n;
+    for (int r : pref) {
+        while (m % r == 0 && m > 1) {
+            if (nrad >= kMaxRadices) return false;
+            rad[nrad++] = r;
+            m /= r;
+        }
+    }
+    return m == 1;
+}
+
+template <typename T>
+int upload_real(const std::vector<double>& h, void** dev) {
+    std::vector<T> tmp(h.begin(), h.end());
+    MMX_HIP_CHECK(hipMalloc(dev, tmp.size() * sizeof(T)));
+    MMX_HIP_CHECK(hipMemcpy(*dev, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MMX_OK;
+}
+
+template <typename T>
+int make_twiddles(int N, void** dev) {
+    std::vector<cx<T>> tw(N);
+    for (int t = 0; t < N; ++t) {
+        // exp(-2 pi i t / N) evaluated in double; sin/cos of the reduced angle keep symmetry exact
+        double ang = -2.0 * M_PI * double(t) / double(N);
+        tw[t] = cx<T>{T(cos(ang)), T(sin(ang))};
+    }
+    MMX_HIP_CHECK(hipMalloc(dev, N * sizeof(cx<T>)));
+    MMX_HIP_CHECK(hipMemcpy(*dev, tw.data(), N * sizeof(cx<T>), hipMemcpyHostToDevice));
+    return MMX_OK;
+}
+
+int ensure_scratch(mmx_plan_s* p, size_t bytes) {
+    if (p->scratch_bytes >= bytes) return MMX_OK;
+    if (p->scratch) MMX_HIP_CHECK(hipFree(p->scratch));
+    p->scratch = nullptr;
+    p->scratch_bytes = 0;
+    MMX_HIP_CHECK(hipMalloc(&p->scratch, bytes));
+    p->scratch_bytes = bytes;
+    return MMX_OK;
+}
+
+int pow2floor(int64_t x) {
+    int r = 1;
+    while ((int64_t)r * 2 <= x) r *= 2;
+    return r;
+}
+
+// Tile shape for one pass: pencils per tile P, threads, dynamic LDS bytes, tiles per problem.
+struct TileCfg {
+    int P, nth, pad;
+    int big;                    // 1: the kernel variant with the larger per-thread element budget
+    size_t lds;
+    int64_t nouter;             // outer blocks the pass iterates (folded passes include batch and components)
+    int64_t ntile_inner;
+    int64_t tiles_per_problem;
+    int64_t nblocks;
+};
+
+// fused: tiles hold `ncomp` components of one problem, grid = tiles_per_problem * nbatch.
+// plain: (batch, component, outer) folded into nouter, ncomp = 1, `nbatch` counts fields.
+template <typename T>
+int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_t lds_max, TileCfg& t) {
+    int E = sizeof(T) == 4 ? kElemsF : kElemsD;
+    t.big = 0;
+    if ((int64_t)(fused ? ncomp : 1) * ax.N > (int64_t)1024 * E) {
+        E = sizeof(T) == 4 ? kElemsFBig : kElemsDBig;
+        t.big = 1;
+    }
+    const size_t el = sizeof(cx<T>);
+    const int N = ax.N;
+    const bool rows = ax.inner == 1;
+    if (!fused) ncomp = 1;
+    t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
+    const int64_t mult = fused ? nbatch : 1;  // independent tile groups besides nouter
+    t.pad = rows ? 1 : 0;
+    const size_t per = (size_t)ncomp * (size_t)(N + t.pad) * el;
+    size_t budget = kLdsBudget;
+    if (N * el + per > budget) budget = lds_max;
+    if (N * el + per > budget) return fail(MMX_EUNSUP, "FFT length too large for one LDS tile");
+    int64_t pm = (int64_t)((budget - N * el) / per);
+    const int64_t pmax_thr = (int64_t)1024 * E / ((int64_t)ncomp * N);
+    if (pmax_thr < 1) return fail(MMX_EUNSUP, "FFT length too large for one workgroup");
+    if (pm > pmax_thr) pm = pmax_thr;
+    if (rows) {
+        const int64_t total_rows = t.nouter * mult;
+        int64_t target = 8;
+        const int64_t by_tiles = total_rows / 1024;  // keep >= ~1024 tiles when the field allows it
+        if (by_tiles < target) target = by_tiles < 1 ? 1 : by_tiles;
+        if (pm > target) pm = target;
+        if (pm > t.nouter) pm = t.nouter;
+        t.P = pow2floor(pm < 1 ? 1 : pm);
+        t.ntile_inner = 1;
+        t.tiles_per_problem = (t.nouter + t.P - 1) / t.P;
+    } else {
+        if (pm > 16) pm = 16;
+        t.P = ax.inner < pm ? (int)ax.inner : pow2floor(pm < 1 ? 1 : pm);
+        // few tiles in total: trade segment length for parallelism
+        while (t.P > 2 && t.nouter * mult * ((ax.inner + t.P - 1) / t.P) < 512) t.P /= 2;
+        t.ntile_inner = (ax.inner + t.P - 1) / t.P;
+        t.tiles_per_problem = t.nouter * t.ntile_inner;
+    }
+    const int64_t need = ((int64_t)ncomp * t.P * N + E - 1) / E;
+    t.nth = (int)(((need + 63) / 64) * 64);
+    if (t.nth < 64) t.nth = 64;
+    if (t.nth > 1024) return fail(MMX_EUNSUP, "internal: tile needs more than 1024 threads");
+    t.lds = (size_t)N * el + (size_t)ncomp * t.P * (N + t.pad) * el;
+    t.nblocks = t.tiles_per_problem * mult;
+    return MMX_OK;
+}
+
+template <typename T>
+void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
+    a.N = ax.N;
+    a.nrad = ax.nrad;
+    for (int r = 0; r < kMaxRadices; ++r) a.rad[r] = r < ax.nrad ? ax.rad[r] : 1;
+    a.inner = ax.inner;
+    a.tw = reinterpret_cast<const cx<T>*>(ax.tw);
+    a.n0 = p->shape3[0];
+    a.n1 = p->shape3[1];
+    a.n2 = p->shape3[2];
+    a.kvec = reinterpret_cast<const T*>(p->kvec);
+    a.k1ofs = p->k1ofs;
+    a.k2ofs = p->k2ofs;
+    a.nd_data = p->ndim;
+    a.inv_ntotal = T(1.0 / double(p->V));
+    a.comp_stride = p->V;
+}
+
+// Launch one pencil pass (see tile_config for `fused`).
+template <typename T, int KIND, int E>
+void launch_kind_e(const TileCfg& t, const PassArgs<T>& a, hipStream_t stream) {
+    if (t.lds > 65536) {
+        static bool raised = false;  // once per instantiation
+        if (!raised) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pencil_pass_kernel<T, E, KIND>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)t.lds);
+            raised = true;
+        }
+    }
+    hipLaunchKernelGGL((pencil_pass_kernel<T, E, KIND>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, stream, a);
+}
+
+template <typename T, int KIND>
+void launch_kind(const TileCfg& t, const PassArgs<T>& a, hipStream_t stream) {
+    constexpr int E = sizeof(T) == 4 ? kElemsF : kElemsD;
+    constexpr int EB = sizeof(T) == 4 ? kElemsFBig : kElemsDBig;
+    if (t.big) launch_kind_e<T, KIND, EB>(t, a, stream);
+    else launch_kind_e<T, KIND, E>(t, a, stream);
+}
+
+template <typename T>
+int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
+                hipStream_t stream) {
+    TileCfg t;
+    int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t);
+    if (rc) return rc;
+    if (!fused) {
+        a.ncomp = 1;
+        a.bstride = 0;
+    }
+    a.nouter = t.nouter;
+    a.P = t.P;
+    a.pad = t.pad;
+    a.ntile_inner = (int)t.ntile_inner;
+    a.tiles_per_problem = t.tiles_per_problem;
+    if (t.nblocks <= 0) return MMX_OK;
+    if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
+    switch (kind) {
+        case kFwd: launch_kind<T, kFwd>(t, a, stream); break;
+        case kInv: launch_kind<T, kInv>(t, a, stream); break;
+        case kProFwd: launch_kind<T, kProFwd>(t, a, stream); break;
+        case kGreen: launch_kind<T, kGreen>(t, a, stream); break;
+        case kInvEpi: launch_kind<T, kInvEpi>(t, a, stream); break;
+        case kAll: launch_kind<T, kAll>(t, a, stream); break;
+        default: return fail(MMX_EINVAL, "internal: unknown pass kind");
+    }
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+template <typename T>
+PassArgs<T> blank_args() {
+    PassArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    a.scale = T(1);
+    a.dir = -1;
+    return a;
+}
+
+template <typename T>
+int fft_impl(mmx_plan_s* p, const void* in, void* out, int64_t nbatch, int direction, hipStream_t stream) {
+    if (nbatch <= 0) return MMX_OK;
+    if (p->naxes == 0) {
+        if (in != out) MMX_HIP_CHECK(hipMemcpyAsync(out, in, nbatch * p->V * sizeof(cx<T>), hipMemcpyDeviceToDevice, stream));
+        return MMX_OK;
+    }
+    const void* src = in;
+    for (int q = p->naxes - 1; q >= 0; --q) {
+        PassArgs<T> a = blank_args<T>();
+        fill_geometry<T>(p, p->ax[q], a);
+        a.src = reinterpret_cast<const cx<T>*>(src);
+        a.dst = reinterpret_cast<cx<T>*>(out);
+        a.dir = direction < 0 ? -1 : 1;
+        a.scale = (direction > 0 && q == 0) ? T(1.0 / double(p->V)) : T(1);
+        int rc = launch_pass<T>(p, p->ax[q], a, nbatch, false, direction < 0 ? kFwd : kInv, stream);
+        if (rc) return rc;
+        src = out;
+    }
+    return MMX_OK;
+}
+
+template <typename T>
+int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* chi, int C, int chi_kind,
+              const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
+              hipStream_t stream) {
+    if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
+    if (chi_kind != 1 && chi_kind != 2) return fail(MMX_EINVAL, "chi_kind must be 1 or 2");
+    if (chi_kind == 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
+    if (nbatch <= 0) return MMX_OK;
+    if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
+    const int d = p->naxes;
+    const AxisPlan& inner = p->ax[d - 1];
+    // partial sums: one pair per tile of the last pass
+    TileCfg tl;
+    int rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl);
+    if (rc) return rc;
+    const int64_t tiles_last = tl.tiles_per_problem;
+    rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
+    if (rc) return rc;
+    double* partials = reinterpret_cast<double*>(p->scratch);
+
+    PassArgs<T> base = blank_args<T>();
+    base.C = C;
+    base.chi_kind = chi_kind;
+    base.E = reinterpret_cast<const cx<T>*>(E);
+    base.chi = reinterpret_cast<const cx<T>*>(chi);
+    base.s = reinterpret_cast<const cx<T>*>(s);
+    base.E_bstride = (int64_t)C * p->V;
+    base.s_bstride = s_bstride;
+    base.chi_bstride = chi_bstride;
+    base.Eout = reinterpret_cast<cx<T>*>(Eout);
+    base.partials = partials;
+    base.coef = reinterpret_cast<const Coef<T>*>(coef);
+    base.vectorial = (C == 3);
+    cx<T>* W = reinterpret_cast<cx<T>*>(Eout);
+
+    if (d == 1) {
+        PassArgs<T> a = base;
+        fill_geometry<T>(p, inner, a);
+        a.prologue = 1;
+        a.green = 1;
+        a.epilogue = 1;
+        a.ncomp = C;
+        a.bstride = (int64_t)C * p->V;
+        a.src = W;
+        a.dst = W;
+        rc = launch_pass<T>(p, inner, a, nbatch, true, kAll, stream);
+        if (rc) return rc;
+    } else {
+        {   // pass 1: prologue + forward FFT along the innermost axis
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, inner, a);
+            a.prologue = 1;
+            a.dir = -1;
+            a.ncomp = C;
+            a.bstride = (int64_t)C * p->V;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, inner, a, nbatch, true, kProFwd, stream);
+            if (rc) return rc;
+        }
+        for (int q = d - 2; q >= 1; --q) {  // middle axes, forward
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[q], a);
+            a.dir = -1;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[q], a, nbatch * C, false, kFwd, stream);
+            if (rc) return rc;
+        }
+        {   // outermost axis: forward FFT, Green's operator, inverse FFT
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[0], a);
+            a.green = 1;
+            a.ncomp = C;
+            a.bstride = (int64_t)C * p->V;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[0], a, nbatch, true, kGreen, stream);
+            if (rc) return rc;
+        }
+        for (int q = 1; q <= d - 2; ++q) {  // middle axes, inverse (1/N folded into G)
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[q], a);
+            a.dir = 1;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[q], a, nbatch * C, false, kInv, stream);
+            if (rc) return rc;
+        }
+        {   // last pass: inverse FFT along the innermost axis + Born update + norms
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, inner, a);
+            a.dir = 1;
+            a.epilogue = 1;
+            a.ncomp = C;
+            a.bstride = (int64_t)C * p->V;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, inner, a, nbatch, true, kInvEpi, stream);
+            if (rc) return rc;
+        }
+    }
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nbatch), dim3(256), 0, stream, partials, tiles_last, norms);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+template <typename T>
+int norm2_impl(mmx_plan_s* p, const void* x, int64_t n, double* out, hipStream_t stream) {
+    const int nblk = 1024;
+    int rc = ensure_scratch(p, nblk * sizeof(double));
+    if (rc) return rc;
+    double* partials = reinterpret_cast<double*>(p->scratch);
+    hipLaunchKernelGGL((norm2_partial_kernel<T>), dim3(nblk), dim3(256), 0, stream,
+                       reinterpret_cast<const cx<T>*>(x), n, partials);
+    MMX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, stream, partials, (int64_t)nblk, out);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+int grid_for(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+template <typename T>
+int chi_mul_impl(mmx_plan_s* p, const void* chi, int chi_kind, const void* x, void* y, int C, int64_t nbatch,
+                 hipStream_t stream) {
+    if (chi_kind == 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
+    if (chi_kind != 1 && chi_kind != 2) return fail(MMX_EINVAL, "chi_kind must be 1 or 2");
+    hipLaunchKernelGGL((chi_mul_kernel<T>), dim3(grid_for(p->V * nbatch)), dim3(256), 0, stream,
+                       reinterpret_cast<const cx<T>*>(chi), chi_kind, reinterpret_cast<const cx<T>*>(x),
+                       reinterpret_cast<cx<T>*>(y), C, p->V, nbatch);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+template <typename T>
+int lproj_impl(mmx_plan_s* p, const void* F, void* out, hipStream_t stream) {
+    hipLaunchKernelGGL((longitudinal_projection_kernel<T>), dim3(grid_for(p->V)), dim3(256), 0, stream,
+                       reinterpret_cast<const cx<T>*>(F), reinterpret_cast<cx<T>*>(out), p->shape3[0],
+                       p->shape3[1], p->shape3[2], reinterpret_cast<const T*>(p->kvec), p->k1ofs, p->k2ofs,
+                       p->ndim);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mmx_version(void) { return 100; }
+
+const char* mmx_last_error(void) { return g_last_error.c_str(); }
+
+int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const double* step, int dtype) {
+    g_last_error.clear();
+    if (!out || !shape || !step) return fail(MMX_EINVAL, "null argument");
+    *out = nullptr;
+    if (ndim < 1 || ndim > 3) return fail(MMX_EINVAL, "grid ndim must be 1, 2 or 3");
+    if (dtype != MMX_C64 && dtype != MMX_C128) return fail(MMX_EINVAL, "dtype must be MMX_C64 or MMX_C128");
+    mmx_plan_s* p = new mmx_plan_s();
+    memset(p, 0, sizeof(*p));
+    p->dtype = dtype;
+    p->ndim = ndim;
+    for (int a = 0; a < 3; ++a) p->shape3[a] = a < ndim ? shape[a] : 1;
+    p->V = p->shape3[0] * p->shape3[1] * p->shape3[2];
+    for (int a = 0; a < 3; ++a) {
+        if (p->shape3[a] < 1) { delete p; return fail(MMX_EINVAL, "shape entries must be >= 1"); }
+    }
+    (void)hipGetDevice(&p->device);
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device) != hipSuccess || v <= 0)
+            v = 65536;
+        p->lds_max = (size_t)v;
+    }
+    // non-trivial axes, outermost first
+    p->naxes = 0;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n = p->shape3[a];
+        if (n == 1) continue;
+        if (n > kMaxN) { delete p; return fail(MMX_EUNSUP, "axis longer than 4096 samples"); }
+        AxisPlan& ax = p->ax[p->naxes++];
+        ax.N = (int)n;
+        if (!factorize(ax.N, ax.rad, ax.nrad)) {
+            delete p;
+            return fail(MMX_EUNSUP, "FFT length " + std::to_string(n) + " has a prime factor above 7");
+        }
+        int64_t inner = 1, outer = 1;
+        for (int b = a + 1; b < 3; ++b) inner *= p->shape3[b];
+        for (int b = 0; b < a; ++b) outer *= p->shape3[b];
+        ax.inner = inner;
+        ax.nouter = outer;
+        int rc = dtype == MMX_C64 ? make_twiddles<float>(ax.N, &ax.tw) : make_twiddles<double>(ax.N, &ax.tw);
+        if (rc) { mmx_plan_destroy(p); return rc; }
+    }
+    // k-vectors: 2 pi fftfreq(n, d) in the order of numpy.fft.fftfreq
+    std::vector<double> k;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n = p->shape3[a];
+        const double d = a < ndim ? step[a] : 1.0;
+        const double val = 1.0 / (double(n) * d);
+        const int64_t npos = (n - 1) / 2 + 1;
+        for (int64_t i = 0; i < n; ++i) {
+            const double f = (i < npos ? double(i) : double(i - n)) * val;
+            k.push_back(2.0 * M_PI * f);
+        }
+    }
+    p->k1ofs = (int)p->shape3[0];
+    p->k2ofs = (int)(p->shape3[0] + p->shape3[1]);
+    int rc = dtype == MMX_C64 ? upload_real<float>(k, &p->kvec) : upload_real<double>(k, &p->kvec);
+    if (rc) { mmx_plan_destroy(p); return rc; }
+    *out = p;
+    return MMX_OK;
+}
+
+int mmx_plan_destroy(mmx_plan_t p) {
+    if (!p) return MMX_OK;
+    for (int a = 0; a < p->naxes; ++a)
+        if (p->ax[a].tw) (void)hipFree(p->ax[a].tw);
+    if (p->kvec) (void)hipFree(p->kvec);
+    if (p->scratch) (void)hipFree(p->scratch);
+    delete p;
+    return MMX_OK;
+}
+
+int mmx_fft(mmx_plan_t p, const void* in, void* out, int64_t nbatch, int direction, void* stream) {
+    g_last_error.clear();
+    if (!p || !in || !out) return fail(MMX_EINVAL, "null argument");
+    if (direction != -1 && direction != 1) return fail(MMX_EINVAL, "direction must be -1 or +1");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64 ? fft_impl<float>(p, in, out, nbatch, direction, s)
+                               : fft_impl<double>(p, in, out, nbatch, direction, s);
+}
+
+int mmx_born_step(mmx_plan_t p, const void* E, void* Eout, const void* s, const void* chi, int ncomp, int chi_kind,
+                  const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
+                  void* stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !s || !chi || !coef || !norms) return fail(MMX_EINVAL, "null argument");
+    if (E == Eout) return fail(MMX_EINVAL, "E and Eout must be different buffers");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64
+               ? born_impl<float>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
+               : born_impl<double>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
+}
+
+int mmx_norm2(mmx_plan_t p, const void* x, int64_t n, double* out, void* stream) {
+    g_last_error.clear();
+    if (!p || !x || !out) return fail(MMX_EINVAL, "null argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64 ? norm2_impl<float>(p, x, n, out, s) : norm2_impl<double>(p, x, n, out, s);
+}
+
+int mmx_chi_mul(mmx_plan_t p, const void* chi, int chi_kind, const void* x, void* y, int ncomp, int64_t nbatch,
+                void* stream) {
+    g_last_error.clear();
+    if (!p || !chi || !x || !y) return fail(MMX_EINVAL, "null argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64 ? chi_mul_impl<float>(p, chi, chi_kind, x, y, ncomp, nbatch, s)
+                               : chi_mul_impl<double>(p, chi, chi_kind, x, y, ncomp, nbatch, s);
+}
+
+int mmx_longitudinal_projection_ft(mmx_plan_t p, const void* F, void* out, void* stream) {
+    g_last_error.clear();
+    if (!p || !F || !out) return fail(MMX_EINVAL, "null argument");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64 ? lproj_impl<float>(p, F, out, s) : lproj_impl<double>(p, F, out, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,416 @@
+// Kernels of the Born-series iteration on gfx950.
+//
+// One iteration d = chi [G (chi E + s) - E], E' = E + d (MacroMax solver.py:1070-1085) runs as
+// 2*ndim - 1 "pencil passes" over the field, each reading and writing the field once:
+//
+//   pass 1         innermost axis, forward FFT, prologue  x = chi E + s            (rows tile)
+//   passes 2..d-1  the middle axes, forward FFT                                     (strided tile)
+//   pass d         outermost axis: forward FFT -> Green's multiply -> inverse FFT   (strided tile)
+//   passes ..      the middle axes, inverse FFT                                     (strided tile)
+//   pass 2d-1      innermost axis, inverse FFT, epilogue d = chi (W - E), E' = E + d,
+//                  FP64 partial sums of |d|^2 and |E'|^2 per workgroup              (rows tile)
+//
+// A tile holds P pencils of every component it needs in LDS; all FFT stages of that axis run in LDS.
+// The 1/N of the inverse transform is folded into the Green's multiply.
+#pragma once
+#include "mmx_device.hpp"
+
+namespace mmx {
+
+// Per-problem preconditioner constants. chi_eff = a chi + c I, s_eff = a s, alpha for G.
+// After a rejected update the host (or the device) changes alpha, a and c instead of rewriting chi
+// (MacroMax solver.py:514-528 rescales chi and s in memory; the affine form is the same operator).
+template <typename T>
+struct Coef {
+    T alpha_r, alpha_i;
+    T a;
+    T c_r, c_i;
+    T pad_;
+};
+
+template <typename T>
+struct PassArgs {
+    // axis geometry (per component, per problem)
+    int N;
+    int nrad;
+    int rad[kMaxRadices];
+    int64_t inner;            // element stride of the axis (1: rows)
+    int64_t nouter;           // outer blocks per component per problem (rows: number of rows)
+    int P;                    // pencils per tile and component
+    int ntile_inner;          // strided tiles along the inner extent
+    int64_t tiles_per_problem;
+    int ncomp;                // components held per tile (1 when components are folded into nouter)
+    int64_t comp_stride;      // V, elements between components
+    int64_t bstride;          // elements between problems (C*V) for src/dst in fused passes
+    int pad;                  // rows layout padding (complex elements)
+    const cx<T>* tw;          // forward twiddles exp(-2 pi i t / N), t < N
+    const cx<T>* src;
+    cx<T>* dst;
+    int dir;                  // plain passes: -1 forward, +1 inverse
+    T scale;                  // plain passes: output scale
+    int prologue;             // 0: load src; 1: x = chi_eff E + s_eff
+    int green;                // 1: forward FFT, Green's multiply, inverse FFT
+    int epilogue;             // 0: store dst; 1: Born update into Eout + norms
+    // Born operands
+    int C;                    // field components (1 or 3)
+    int chi_kind;             // 1: [1,1,*g]; 2: [3,3,*g]
+    const cx<T>* E;
+    const cx<T>* chi;
+    const cx<T>* s;
+    int64_t E_bstride, chi_bstride, s_bstride;
+    cx<T>* Eout;
+    double* partials;         // 2 per (problem, tile)
+    const Coef<T>* coef;      // one per problem
+    // Green's function
+    int64_t n0, n1, n2;       // grid shape padded with trailing 1s
+    const T* kvec;            // k_0 at [0, n0), k_1 at [k1ofs, +n1), k_2 at [k2ofs, +n2)
+    int k1ofs, k2ofs;
+    int nd_data;              // grid ndim
+    int vectorial;
+    T inv_ntotal;
+};
+
+// Green's operator at one k for the C components held in f[] (MacroMax solver.py:566-581 with
+// backend/__init__.py:606-649): g = -i alpha_i / (k^2 - alpha);
+// vectorial: F <- g (F - Pi_L F) + (i alpha_i / alpha) Pi_L F, Pi_L = k k^T / k^2, Pi_L(0) = I on the
+// first nd_data components and 0 on the others. The result is scaled by 1/N_total.
+template <typename T>
+__device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, bool dc, const PassArgs<T>& a,
+                                            const Coef<T>& cf) {
+    const cx<T> alpha{cf.alpha_r, cf.alpha_i};
+    const T kk = k0 * k0 + k1 * k1 + k2 * k2;
+    // g = -i alpha_i / (kk - alpha), times 1/N
+    const cx<T> den{kk - alpha.r, -alpha.i};
+    const cx<T> num{T(0), -cf.alpha_i * a.inv_ntotal};
+    const cx<T> g = cdiv(num, den);
+    if (!a.vectorial || C == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) if (c < C) f[c] = g * f[c];
+        return;
+    }
+    // i alpha_i / alpha, times 1/N
+    const cx<T> gl = cdiv(cx<T>{T(0), cf.alpha_i * a.inv_ntotal}, alpha);
+    cx<T> pl[3];
+    if (dc) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pl[c] = (c < a.nd_data) ? f[c] : cx<T>{T(0), T(0)};
+    } else {
+        const T inv = T(1) / kk;
+        cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
+        proj = scal(proj, inv);
+        pl[0] = scal(proj, k0);
+        pl[1] = scal(proj, k1);
+        pl[2] = scal(proj, k2);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = g * (f[c] - pl[c]) + gl * pl[c];
+}
+
+template <typename T>
+__device__ __forceinline__ void unravel3(int64_t v, const PassArgs<T>& a, int64_t& i0, int64_t& i1, int64_t& i2) {
+    i2 = v % a.n2;
+    const int64_t t = v / a.n2;
+    i1 = t % a.n1;
+    i0 = t / a.n1;
+}
+
+// chi_eff applied to the C-vector x at voxel v of problem b.
+template <typename T>
+__device__ __forceinline__ void chi_apply(const PassArgs<T>& a, const cx<T>* chi_b, int64_t v, const cx<T>* x,
+                                          cx<T>* y, const Coef<T>& cf) {
+    const cx<T> cc{cf.c_r, cf.c_i};
+    if (a.chi_kind == 1) {
+        const cx<T> x0 = chi_b[v];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = scal(x0 * x[c], cf.a) + cc * x[c];
+    } else {
+        const int64_t V = a.comp_stride;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            cx<T> acc = chi_b[(r * 3 + 0) * V + v] * x[0];
+            acc = acc + chi_b[(r * 3 + 1) * V + v] * x[1];
+            acc = acc + chi_b[(r * 3 + 2) * V + v] * x[2];
+            y[r] = scal(acc, cf.a) + cc * x[r];
+        }
+    }
+}
+
+// Pass kinds: which prologue / transform / epilogue a pencil pass runs.
+enum PassKind : int {
+    kFwd = 0,     // load, forward FFT, store
+    kInv = 1,     // load, inverse FFT, store * scale
+    kProFwd = 2,  // Born prologue (chi E + s), forward FFT, store
+    kGreen = 3,   // load, forward FFT, Green's operator, inverse FFT, store
+    kInvEpi = 4,  // load, inverse FFT, Born epilogue (update + norms)
+    kAll = 5,     // Born prologue, forward, Green, inverse, Born epilogue (single-axis grids)
+};
+
+template <typename T, int E, int KIND>
+__global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
+    constexpr bool PRO = (KIND == kProFwd || KIND == kAll);
+    constexpr bool GREEN = (KIND == kGreen || KIND == kAll);
+    constexpr bool EPI = (KIND == kInvEpi || KIND == kAll);
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double red[16];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tile = tw + a.N;
+    const int tid = threadIdx.x;
+    const int nth = blockDim.x;
+    const int N = a.N;
+    const bool rows = (a.inner == 1);
+
+    // which problem and which tile
+    const int64_t blk = blockIdx.x;
+    const int64_t b = blk / a.tiles_per_problem;
+    const int64_t t = blk % a.tiles_per_problem;
+    const Coef<T> cf = (a.coef != nullptr) ? a.coef[b] : Coef<T>{T(0), T(1), T(1), T(0), T(0), T(0)};
+
+    TileGeom g;
+    g.N = N;
+    g.M = a.ncomp * a.P;
+    if (rows) { g.SJ = 1; g.SM = N + a.pad; g.m_fast = 0; }
+    else { g.SJ = g.M; g.SM = 1; g.m_fast = 1; }
+
+    for (int q = tid; q < N; q += nth) tw[q] = a.tw[q];
+
+    // tile origin: rows -> first row r0; strided -> (outer block o, first inner index i0)
+    int64_t r0 = 0, o = 0, ib = 0;
+    if (rows) r0 = t * a.P;
+    else { o = t / a.ntile_inner; ib = (t % a.ntile_inner) * a.P; }
+    const int64_t pb = b * a.bstride;  // problem offset in src/dst (fused passes; 0 when folded)
+
+    // ---------------- load ----------------
+    if constexpr (PRO) {
+        // rows tile with all C components; voxel (row, j)
+        const cx<T>* Eb = a.E + b * a.E_bstride;
+        const cx<T>* sb = a.s + b * a.s_bstride;
+        const cx<T>* chib = a.chi + b * a.chi_bstride;
+        const int64_t V = a.comp_stride;
+        const int nvox = a.P * N;
+        for (int idx = tid; idx < nvox; idx += nth) {
+            const int j = idx % N;
+            const int p = idx / N;
+            const int64_t row = r0 + p;
+            cx<T> y[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+            if (row < a.nouter) {
+                const int64_t v = row * N + j;
+                cx<T> x[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) if (c < a.C) x[c] = Eb[c * V + v];
+                chi_apply<T>(a, chib, v, x, y, cf);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = y[c] + scal(sb[c * V + v], cf.a);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) if (c < a.C) tile[(c * a.P + p) * g.SM + j] = y[c];
+        }
+    } else {
+      if (rows) {
+        const int total = a.ncomp * a.P * N;
+        for (int idx = tid; idx < total; idx += nth) {
+            const int j = idx % N;
+            const int rest = idx / N;
+            const int p = rest % a.P;
+            const int c = rest / a.P;
+            const int64_t row = r0 + p;
+            cx<T> val{T(0), T(0)};
+            if (row < a.nouter) val = a.src[pb + c * a.comp_stride + row * N + j];
+            tile[(c * a.P + p) * g.SM + j] = val;
+        }
+    } else {
+        const int total = a.ncomp * a.P * N;
+        const int64_t obase = pb + o * (int64_t)N * a.inner + ib;
+        for (int idx = tid; idx < total; idx += nth) {
+            const int p = idx % a.P;
+            const int rest = idx / a.P;
+            const int c = rest % a.ncomp;
+            const int j = rest / a.ncomp;
+            cx<T> val{T(0), T(0)};
+            if (ib + p < a.inner) val = a.src[obase + c * a.comp_stride + j * a.inner + p];
+            tile[idx] = val;  // == j*M + c*P + p
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---------------- FFT (and Green) ----------------
+    if constexpr (GREEN) {
+        tile_fft<T, -1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+        const int nvox = a.P * N;
+        for (int idx = tid; idx < nvox; idx += nth) {
+            int p, j;
+            int64_t v;
+            if (rows) { j = idx % N; p = idx / N; v = (r0 + p) * N + j; if (r0 + p >= a.nouter) continue; }
+            else { p = idx % a.P; j = idx / a.P; if (ib + p >= a.inner) continue; v = (o * N + j) * a.inner + ib + p; }
+            int64_t i0, i1, i2;
+            unravel3(v, a, i0, i1, i2);
+            const T k0 = a.kvec[i0];
+            const T k1 = (a.n1 > 1) ? a.kvec[a.k1ofs + i1] : T(0);
+            const T k2 = (a.n2 > 1) ? a.kvec[a.k2ofs + i2] : T(0);
+            cx<T> f[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) if (c < a.ncomp) f[c] = tile[j * g.SJ + (c * a.P + p) * g.SM];
+            green_apply<T>(f, a.ncomp, k0, k1, k2, v == 0, a, cf);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) if (c < a.ncomp) tile[j * g.SJ + (c * a.P + p) * g.SM] = f[c];
+        }
+        __syncthreads();
+        tile_fft<T, +1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+    } else if constexpr (KIND == kFwd || KIND == kProFwd) {
+        tile_fft<T, -1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+    } else {
+        tile_fft<T, +1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+    }
+
+    // ---------------- store ----------------
+    if constexpr (EPI) {
+        // rows tile with all C components: d = chi_eff (W - E), E' = E + d
+        const cx<T>* Eb = a.E + b * a.E_bstride;
+        const cx<T>* chib = a.chi + b * a.chi_bstride;
+        cx<T>* Eo = a.Eout + b * a.E_bstride;
+        const int64_t V = a.comp_stride;
+        double dd = 0.0, ee = 0.0;
+        const int nvox = a.P * N;
+        for (int idx = tid; idx < nvox; idx += nth) {
+            const int j = idx % N;
+            const int p = idx / N;
+            const int64_t row = r0 + p;
+            if (row >= a.nouter) continue;
+            const int64_t v = row * N + j;
+            const cx<T> z{T(0), T(0)};
+            cx<T> e[3] = {z, z, z}, w[3] = {z, z, z}, d[3] = {z, z, z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) if (c < a.C) {
+                e[c] = Eb[c * V + v];
+                w[c] = tile[(c * a.P + p) * g.SM + j] - e[c];
+            }
+            chi_apply<T>(a, chib, v, w, d, cf);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) if (c < a.C) {
+                const cx<T> en = e[c] + d[c];
+                Eo[c * V + v] = en;
+                dd += norm2_f64(d[c]);
+                ee += norm2_f64(en);
+            }
+        }
+        dd = block_sum(dd, red);
+        ee = block_sum(ee, red);
+        if (tid == 0) {
+            a.partials[2 * blk + 0] = dd;
+            a.partials[2 * blk + 1] = ee;
+        }
+    } else {
+      if (rows) {
+        const int total = a.ncomp * a.P * N;
+        for (int idx = tid; idx < total; idx += nth) {
+            const int j = idx % N;
+            const int rest = idx / N;
+            const int p = rest % a.P;
+            const int c = rest / a.P;
+            const int64_t row = r0 + p;
+            if (row < a.nouter) a.dst[pb + c * a.comp_stride + row * N + j] = scal(tile[(c * a.P + p) * g.SM + j], a.scale);
+        }
+    } else {
+        const int total = a.ncomp * a.P * N;
+        const int64_t obase = pb + o * (int64_t)N * a.inner + ib;
+        for (int idx = tid; idx < total; idx += nth) {
+            const int p = idx % a.P;
+            const int rest = idx / a.P;
+            const int c = rest % a.ncomp;
+            const int j = rest / a.ncomp;
+            if (ib + p < a.inner) a.dst[obase + c * a.comp_stride + j * a.inner + p] = scal(tile[idx], a.scale);
+        }
+      }
+    }
+}
+
+// Sum of per-tile partials -> out[2*b + {0,1}], fixed order (deterministic). One block per problem.
+__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
+    __shared__ double red[16];
+    const int64_t b = blockIdx.x;
+    const double* pp = partials + 2 * b * tiles;
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t q = threadIdx.x; q < tiles; q += blockDim.x) {
+        s0 += pp[2 * q];
+        s1 += pp[2 * q + 1];
+    }
+    s0 = block_sum(s0, red);
+    s1 = block_sum(s1, red);
+    if (threadIdx.x == 0) {
+        out[2 * b] = s0;
+        out[2 * b + 1] = s1;
+    }
+}
+
+// sum |x|^2 in FP64: per-block partials (pass 1), then reduce_partials-style sum (pass 2).
+template <typename T>
+__global__ void __launch_bounds__(256) norm2_partial_kernel(const cx<T>* x, int64_t n, double* partials) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        s += norm2_f64(x[q]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256) sum_kernel(const double* partials, int64_t n, double* out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x) s += partials[q];
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// y = chi x per voxel (chi [1,1,V] or [3,3,V]; x, y [C,V] with batch strides), no affine coefficients.
+template <typename T>
+__global__ void __launch_bounds__(256) chi_mul_kernel(const cx<T>* chi, int chi_kind, const cx<T>* x, cx<T>* y,
+                                                      int C, int64_t V, int64_t nbatch) {
+    const int64_t total = V * nbatch;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = q / V;
+        const int64_t v = q % V;
+        const cx<T>* xb = x + b * C * V;
+        cx<T>* yb = y + b * C * V;
+        if (chi_kind == 1) {
+            const cx<T> c0 = chi[v];
+            for (int c = 0; c < C; ++c) yb[c * V + v] = c0 * xb[c * V + v];
+        } else {
+            cx<T> xv[3] = {xb[v], xb[V + v], xb[2 * V + v]};
+            cx<T> yv[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                yv[r] = chi[(r * 3) * V + v] * xv[0] + chi[(r * 3 + 1) * V + v] * xv[1] + chi[(r * 3 + 2) * V + v] * xv[2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) yb[r * V + v] = yv[r];
+        }
+    }
+}
+
+// Pi_L F at every k (MacroMax backend/__init__.py:606-649). F, out: [3, V] (out may alias F).
+template <typename T>
+__global__ void __launch_bounds__(256) longitudinal_projection_kernel(const cx<T>* F, cx<T>* out, int64_t n0,
+                                                                      int64_t n1, int64_t n2, const T* kvec,
+                                                                      int k1ofs, int k2ofs, int nd_data) {
+    const int64_t V = n0 * n1 * n2;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < V; v += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i2 = v % n2, t = v / n2, i1 = t % n1, i0 = t / n1;
+        const T k0 = kvec[i0];
+        const T k1 = (n1 > 1) ? kvec[k1ofs + i1] : T(0);
+        const T k2 = (n2 > 1) ? kvec[k2ofs + i2] : T(0);
+        cx<T> f[3] = {F[v], F[V + v], F[2 * V + v]};
+        cx<T> pl[3];
+        if (v == 0) {
+            for (int c = 0; c < 3; ++c) pl[c] = (c < nd_data) ? f[c] : cx<T>{T(0), T(0)};
+        } else {
+            const T kk = k0 * k0 + k1 * k1 + k2 * k2;
+            cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
+            proj = scal(proj, T(1) / kk);
+            pl[0] = scal(proj, k0);
+            pl[1] = scal(proj, k1);
+            pl[2] = scal(proj, k2);
+        }
+        for (int c = 0; c < 3; ++c) out[c * V + v] = pl[c];
+    }
+}
+
+}  // namespace mmx

@@ -1,0 +1,482 @@
+"""Solve the macroscopic Maxwell / Helmholtz equations with the modified Born series on an MI355X.
+
+Mirrors MacroMax ``solver.py``: ``solve()`` (``:25-89``) and ``Solution`` (``:92-1123``) with the same constructor
+arguments, properties and iteration semantics. The iteration ``d = chi [G(chi E + s) - E]``, the accept test
+``||d|| < ||d_prev||`` and the divergence guard ``alpha <- alpha_r + 1.5 i alpha_i`` follow ``:1049-1098``; one
+iteration is a single fused sequence of gfx950 kernels (``BackEndHip.born_step``) with one host read of two FP64
+norms. The electric field is double-buffered: the kernels write ``E + d`` into the second buffer, which becomes
+the field only when the update is accepted.
+
+Scope: non-magnetic media (``mu`` a scalar, ``xi = zeta = 0``), isotropic or anisotropic (3x3) permittivity,
+scalar or vectorial fields on 1-, 2- and 3-dimensional grids. Magnetic and bi-anisotropic media raise
+``NotImplementedError``.
+"""
+from __future__ import annotations
+
+import logging
+import math
+from collections.abc import Callable
+
+import numpy as np
+import scipy.constants as const
+import scipy.optimize
+import torch
+
+from . import backend
+from .bound import Bound, Electric, Magnetic, PeriodicBound
+from .grid import Grid
+
+log = logging.getLogger(__name__)
+
+__all__ = ['solve', 'Solution']
+
+
+def solve(grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_wavelength=None,
+          current_density=None, source_distribution=None, epsilon=None, xi=0.0, zeta=0.0, mu=1.0,
+          refractive_index=None, bound=None, initial_field=0.0, dtype=None,
+          callback: Callable = lambda s: s.iteration < 1e4 and s.residue > 1e-4, rtol: float = 0,
+          maxiter=math.inf):
+    """Construct a Solution and iterate it until ``callback`` returns False (``solver.py:25-89``)."""
+    if callback is None:
+        def callback(_):
+            return _.iteration < maxiter and _.residue > rtol
+    return Solution(grid=grid, vectorial=vectorial, wavenumber=wavenumber, angular_frequency=angular_frequency,
+                    vacuum_wavelength=vacuum_wavelength, current_density=current_density,
+                    source_distribution=source_distribution, epsilon=epsilon, xi=xi, zeta=zeta, mu=mu,
+                    refractive_index=refractive_index, bound=bound, initial_field=initial_field,
+                    dtype=dtype).solve(callback)
+
+
+class Solution:
+    """State of a Born-series solution; iterate it with ``solve()`` or ``for s in solution``."""
+
+    def __init__(self, grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_wavelength=None,
+                 current_density=None, source_distribution=None, epsilon=None, xi=0.0, zeta=0.0, mu=1.0,
+                 refractive_index=None, bound: Bound | None = None, initial_field=0.0, dtype=None):
+        self.__iteration = 0
+        if not isinstance(grid, Grid):
+            if np.isscalar(grid[0]):
+                grid = [grid]
+            grid = Grid.from_ranges(*grid)
+        self.__grid = grid.immutable
+
+        if angular_frequency is not None:
+            self.__wavenumber = angular_frequency / const.c
+        elif vacuum_wavelength is not None:
+            self.__wavenumber = 2 * const.pi / vacuum_wavelength
+        else:
+            self.__wavenumber = wavenumber
+
+        self.__previous_update_norm = math.inf
+        self.__field_norm = None
+
+        def func2arr(f):
+            if callable(f):
+                return f(*self.__grid)
+            return f if isinstance(f, torch.Tensor) else np.asarray(f)
+
+        # source (solver.py:174-185)
+        if source_distribution is None:
+            if current_density is None:
+                if vectorial is None:
+                    vectorial = True
+                current_density = np.zeros((1 + 2 * vectorial, *self.grid.shape),
+                                           dtype=dtype if dtype is not None else np.complex128)
+            current_density = func2arr(current_density)
+            source_distribution = current_density * (-1j * self.angular_frequency * const.mu_0)
+        else:
+            source_distribution = func2arr(source_distribution)
+
+        if vectorial is None:
+            vectorial = source_distribution.ndim > self.grid.ndim and source_distribution.shape[0] == 3
+        self.__vectorial = bool(vectorial)
+
+        if bound is None:
+            bound = PeriodicBound(self.grid)
+        self.__bound = bound
+
+        if dtype is None:
+            dtype = np.complex64 if (isinstance(source_distribution, torch.Tensor) and
+                                     source_distribution.dtype in (torch.complex64, torch.float32)) else \
+                (np.complex128 if isinstance(source_distribution, torch.Tensor) else np.asarray(source_distribution).dtype)
+        if not np.issubdtype(dtype, np.complexfloating):
+            dtype = np.complex64 if dtype in (np.float16, np.float32) else np.complex128
+
+        # the backend works in units of 1/k0 (solver.py:212)
+        self.__BE = backend.load(1 + 2 * self.vectorial, self.grid * self.wavenumber, dtype=dtype)
+        BE = self.__BE
+        self.__field_array = BE.allocate_array(fill_value=0)
+        self.__field_next = BE.allocate_array(fill_value=0)
+        self.__norms = torch.zeros(2, dtype=torch.float64, device=BE.device)
+        self.E = initial_field
+
+        mu = func2arr(mu)
+        xi = func2arr(xi)
+        zeta = func2arr(zeta)
+        if np.any(np.asarray(xi) != 0) or np.any(np.asarray(zeta) != 0) or np.asarray(mu).size != 1:
+            raise NotImplementedError('Magnetic and bi-(an)isotropic media are not supported by this package.')
+        self.__mu = complex(np.asarray(mu).ravel()[0])
+        if isinstance(bound, Magnetic) and np.any(np.asarray(bound.magnetic_susceptibility) != 0):
+            raise NotImplementedError('Magnetic boundaries are not supported by this package.')
+
+        if epsilon is None:
+            n = func2arr(refractive_index) if refractive_index is not None else 1.0
+            n = BE.to_matrix_field(np.asarray(n).astype(dtype))
+            epsilon = BE.mul(n, n)
+            if BE.is_scalar(n) and bool(torch.any(n.real < 0)):
+                raise NotImplementedError('Negative refractive indices imply a magnetic medium (unsupported).')
+        else:
+            epsilon = func2arr(epsilon)
+            epsilon = BE.astype(epsilon if isinstance(epsilon, torch.Tensor) else np.asarray(epsilon).astype(dtype))
+
+        # boundary (solver.py:242-247)
+        if isinstance(self.__bound, Electric):
+            chi_b = BE.astype(self.__bound.electric_susceptibility)
+            if epsilon.ndim >= 2 + self.grid.ndim and (epsilon.shape[0] > 1 or epsilon.shape[1] > 1):
+                chi_b = BE.eye * chi_b
+            epsilon = BE.astype(BE.to_matrix_field(epsilon) + chi_b)
+
+        self.__source_normalized = None
+        self.__prepare_preconditioner(source_distribution, epsilon)
+        self.__relative_residue = None
+
+    # ------------------------------------------------------------------ preconditioner (solver.py:270-493)
+    def __prepare_preconditioner(self, source_distribution, epsilon):
+        BE = self.__BE
+        eps = BE.to_matrix_field(epsilon)
+        if eps.shape[0] != eps.shape[1] or eps.shape[0] not in (1, 3):
+            raise ValueError(f'permittivity must be a scalar or 3x3 tensor field, not {tuple(eps.shape[:2])}')
+        if eps.shape[0] == 3 and not self.vectorial:
+            raise ValueError('an anisotropic permittivity needs a vectorial calculation')
+        eps64 = eps.to(torch.complex128)
+        max_allowed_gain = math.sqrt(BE.eps)
+
+        def herm_part_min(a):  # smallest eigenvalue of -0.5i (a - a^H)
+            if a.shape[0] == 1:
+                return float(torch.amin((-0.5j * (a - a.conj())).real))
+            h = -0.5j * (a - a.transpose(0, 1).conj())
+            return float(torch.amin(_eigvals(h)[0]))
+
+        gain = herm_part_min(eps64) < -max_allowed_gain
+        loss = herm_part_min(-eps64) < -max_allowed_gain
+        transpose = gain and not loss
+        if gain and not transpose:
+            log.warning('Convergence not guaranteed: the permittivity has gain.')
+
+        beta = float(np.real(1.0 / self.__mu))
+
+        def sigma_ee(alpha_r):
+            ab = float(alpha_r) * beta
+            if eps64.shape[0] == 1:
+                return float(torch.amax((eps64 - ab).abs())) / abs(beta)
+            d = eps64 - ab * torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(3, 3, *([1] * self.grid.ndim))
+            dh = d.transpose(0, 1).conj()
+            delta2 = torch.einsum('ij...,jk...->ik...', dh, d)
+            lam = _eigvals(delta2)
+            return float(torch.amax(lam.abs())) ** 0.5 / abs(beta)
+
+        alpha_real, min_value = scipy.optimize.fmin(lambda a: sigma_ee(np.asarray(a).ravel()[0]), 0.0,
+                                                    initial_simplex=[[0.0], [1.0]], disp=False, full_output=True,
+                                                    ftol=0.01, xtol=0.01, maxiter=100, maxfun=100)[:2]
+        alpha = complex(alpha_real[0] + 1.0j * min_value)
+        if transpose:
+            alpha = alpha.conjugate()
+        alpha = self.__increase_bias_to_limit_kernel_width(alpha)
+        log.info(f'Preconditioner constants: alpha = {alpha.real:0.4g} + {alpha.imag:0.4g}i, beta = {beta:0.4g}')
+        self.__alpha = alpha
+        self.__beta = beta
+        self.__eps = eps  # without any alpha; chi is derived from it
+        self.__coef_a = 1.0
+        self.__coef_c = 0j
+        self.__alpha_base = alpha
+        # chi = (i/alpha_i)(eps/beta - alpha I), materialised densely for the kernels (solver.py:480-484)
+        full = tuple(int(_) for _ in self.grid.shape)
+        chi = eps64 * (1.0j / alpha.imag / beta)
+        if chi.shape[0] == 1:
+            chi = chi - alpha * 1.0j / alpha.imag
+        else:
+            chi = chi - torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(3, 3, *([1] * self.grid.ndim)) \
+                * (alpha * 1.0j / alpha.imag)
+        self.__chi = chi.expand(chi.shape[0], chi.shape[1], *full).to(BE.hardware_dtype).contiguous()
+        self.__chi_kind = 1 if chi.shape[0] == 1 else 2
+        self.__coef = torch.empty(6, dtype=BE.real_dtype, device=BE.device)
+        self.__upload_coef()
+        self.source_distribution = source_distribution
+
+    def __upload_coef(self):
+        a = self.__alpha
+        vals = [a.real, a.imag, self.__coef_a, self.__coef_c.real, self.__coef_c.imag, 0.0]
+        self.__coef.copy_(torch.tensor(vals, dtype=torch.float64))
+
+    def __update_operators(self, alpha_new: complex):
+        """Change alpha after a rejected update (``solver.py:496-529``) via chi_eff = a chi + c I, s_eff = a s."""
+        a_old = self.__alpha
+        A = a_old.imag / alpha_new.imag
+        B = 1.0j * (a_old - alpha_new) / alpha_new.imag
+        self.__coef_a = self.__coef_a * A
+        self.__coef_c = self.__coef_c * A + B
+        self.__alpha = alpha_new
+        self.__upload_coef()
+
+    def __increase_bias_to_limit_kernel_width(self, alpha, max_kernel_field_residue=0.001):
+        """``solver.py:591-627``."""
+        offset = alpha.imag + 0.05
+        t = self.__bound.thickness
+        t = t[t != 0]
+        if t.size > 0:
+            d_min = np.amin(t)
+            min_kappa = np.log(max_kernel_field_residue) / (-self.wavenumber * d_min)
+            n_real = np.maximum(0, float(alpha.real) + min_kappa ** 2) ** 0.5
+            min_offset = 2 * n_real * min_kappa
+            if offset < min_offset:
+                offset = min_offset
+        return complex(alpha.real, offset)
+
+    # ------------------------------------------------------------------ properties
+    @property
+    def grid(self) -> Grid:
+        return self.__grid
+
+    @property
+    def vectorial(self) -> bool:
+        return self.__vectorial
+
+    @property
+    def dtype(self):
+        return self.__BE.numpy_dtype
+
+    @property
+    def wavenumber(self) -> float:
+        return self.__wavenumber
+
+    @property
+    def angular_frequency(self) -> float:
+        return self.__wavenumber * const.c
+
+    @property
+    def wavelength(self) -> float:
+        return 2.0 * const.pi / self.__wavenumber
+
+    @property
+    def magnetic(self) -> bool:
+        return False
+
+    @property
+    def bound(self) -> Bound:
+        return self.__bound
+
+    @property
+    def backend(self):
+        return self.__BE
+
+    @property
+    def alpha(self) -> complex:
+        """Current preconditioner constant alpha (in units of k0^2)."""
+        return self.__alpha
+
+    @property
+    def beta(self) -> float:
+        return self.__beta
+
+    @property
+    def chi(self) -> torch.Tensor:
+        """The stored susceptibility chi (for the alpha chosen at set-up; see ``chi_effective``)."""
+        return self.__chi
+
+    @property
+    def chi_effective(self) -> torch.Tensor:
+        c = self.__chi * self.__coef_a
+        if self.__coef_c != 0:
+            if c.shape[0] == 1:
+                c = c + self.__coef_c
+            else:
+                c = c + self.__BE.eye * self.__coef_c
+        return c
+
+    @property
+    def source_normalized(self) -> torch.Tensor:
+        """s = (i/(alpha_i beta)) S for the current alpha, [C, 1, *grid]."""
+        return self.__source_normalized * self.__coef_a
+
+    @property
+    def source_distribution(self) -> np.ndarray:
+        s = self.source_normalized[:, 0] * (self.__beta / 1.0j * self.__alpha.imag)
+        return self.__BE.asnumpy(s)
+
+    @source_distribution.setter
+    def source_distribution(self, new_source):
+        """Set S; stores s_base = i S / (alpha_i beta) for the set-up alpha (``solver.py:702-720``).
+
+        The kernels use s_eff = a s_base, which equals i S / (alpha_i beta) for the current alpha because
+        a = alpha_i(set-up) / alpha_i(current).
+        """
+        BE = self.__BE
+        if not isinstance(new_source, torch.Tensor):
+            new_source = np.asarray(new_source).astype(BE.numpy_dtype)
+        S = BE.to_matrix_field(new_source)
+        full = (1 + 2 * self.vectorial, 1, *[int(_) for _ in self.grid.shape])
+        S = S.to(torch.complex128).expand(*full)
+        self.__source_normalized = (S * (1.0j / self.__alpha_base.imag / self.__beta)).to(
+            BE.hardware_dtype).contiguous()
+        self.__previous_update_norm = math.inf
+
+    @property
+    def j(self) -> np.ndarray:
+        return self.source_distribution / (-1.0j * self.angular_frequency * const.mu_0)
+
+    @j.setter
+    def j(self, new_j):
+        self.source_distribution = np.asarray(new_j) * (-1.0j * self.angular_frequency * const.mu_0)
+
+    @property
+    def epsilon(self) -> np.ndarray:
+        """Permittivity without the bound (``solver.py:732-753``)."""
+        eps = self.__eps
+        if isinstance(self.__bound, Electric):
+            chi_b = self.__BE.astype(self.__bound.electric_susceptibility)
+            if eps.shape[0] > 1:
+                chi_b = self.__BE.eye * chi_b
+            eps = eps - chi_b
+        return self.__BE.asnumpy(eps)
+
+    @epsilon.setter
+    def epsilon(self, new_value):
+        """Set the permittivity (bound added) and rebuild the preconditioner (``solver.py:755-787``)."""
+        BE = self.__BE
+        src = self.source_distribution
+        eps = BE.to_matrix_field(np.asarray(new_value).astype(BE.numpy_dtype))
+        if isinstance(self.__bound, Electric):
+            chi_b = BE.astype(self.__bound.electric_susceptibility)
+            if eps.shape[0] > 1 or eps.shape[1] > 1:
+                chi_b = BE.eye * chi_b
+            eps = eps + chi_b
+        self.__prepare_preconditioner(src[:, np.newaxis], eps)
+
+    @property
+    def refractive_index(self) -> np.ndarray:
+        return self.epsilon ** 0.5
+
+    @refractive_index.setter
+    def refractive_index(self, new_value):
+        self.epsilon = np.asarray(new_value) ** 2
+
+    @property
+    def E(self) -> np.ndarray:
+        """Electric field in SI units, [C, *grid] (``solver.py:811-821``)."""
+        return self.__BE.asnumpy(self.__field_array[:, 0] / (self.wavenumber ** 2))
+
+    @E.setter
+    def E(self, E):
+        BE = self.__BE
+        val = E if isinstance(E, torch.Tensor) else np.asarray(E)
+        t = BE.astype(val) * (self.wavenumber ** 2)
+        if t.ndim == self.grid.ndim + 1:  # [C, *grid] -> [C, 1, *grid]
+            t = t.unsqueeze(1)
+        BE.assign(t, self.__field_array)
+        self.__previous_update_norm = math.inf
+        self.__field_norm = None
+
+    @property
+    def field_array(self) -> torch.Tensor:
+        """The k0^2-scaled field on the device, [C, 1, *grid] (read-only view of the current buffer)."""
+        return self.__field_array
+
+    @property
+    def B(self) -> np.ndarray:
+        BE = self.__BE
+        B = BE.curl(BE.astype(self.E[:, np.newaxis])) / (1.0j * const.c)
+        return BE.asnumpy(B)[:, 0]
+
+    @property
+    def H(self) -> np.ndarray:
+        BE = self.__BE
+        mu_H = (-1.0j / (const.mu_0 * const.c)) * BE.curl(BE.astype(self.E[:, np.newaxis]))
+        return BE.asnumpy(mu_H * self.__beta)[:, 0]
+
+    @property
+    def D(self) -> np.ndarray:
+        BE = self.__BE
+        D = BE.curl(BE.astype(self.H[:, np.newaxis])) * self.wavenumber
+        S = BE.astype(self.source_distribution[:, np.newaxis])
+        D = D - (1.0 / (1.0j * self.angular_frequency * const.mu_0)) * S * self.__beta
+        D = D * (1j / self.angular_frequency)
+        return BE.asnumpy(D)[:, 0]
+
+    @property
+    def S(self) -> np.ndarray:
+        BE = self.__BE
+        E = BE.astype(self.E[:, np.newaxis])
+        H = BE.astype(self.H[:, np.newaxis])
+        return (0.5 * BE.asnumpy(BE.cross(E, H.conj())).real)[:, 0]
+
+    @property
+    def iteration(self) -> int:
+        return self.__iteration
+
+    @iteration.setter
+    def iteration(self, it: int = 0):
+        self.__iteration = it
+
+    @property
+    def previous_update_norm(self) -> float:
+        return float(self.__previous_update_norm / (self.wavenumber ** 2))
+
+    @property
+    def relative_residue(self) -> float:
+        """||d_prev|| / ||E|| (``solver.py:1026-1042``)."""
+        if self.__relative_residue is None:
+            if self.__previous_update_norm > 0:
+                if self.__field_norm is None:
+                    self.__field_norm = self.__BE.norm(self.__field_array)
+                self.__relative_residue = self.__previous_update_norm / self.__field_norm
+            else:
+                self.__relative_residue = 0.0
+        return float(self.__relative_residue)
+
+    @property
+    def residue(self) -> float:
+        return self.relative_residue
+
+    # ------------------------------------------------------------------ iteration (solver.py:1049-1098)
+    def step(self) -> bool:
+        """One Born iteration; returns True if the update was accepted."""
+        self.__iteration += 1
+        self.__relative_residue = None
+        BE = self.__BE
+        BE.born_step(self.__field_array, self.__field_next, self.__source_normalized, self.__chi, self.__chi_kind,
+                     self.__coef, self.__norms)
+        d2, e2 = self.__norms.tolist()  # the one host sync of the iteration
+        current = math.sqrt(d2)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            rel = np.float64(current) / np.float64(self.__previous_update_norm)
+        if rel < 1:
+            self.__field_array, self.__field_next = self.__field_next, self.__field_array
+            self.__previous_update_norm = current
+            self.__field_norm = math.sqrt(e2)
+            return True
+        log.warning(f'The field update is scaled by {rel:0.3f} >= 1 in iteration {self.__iteration}.')
+        alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)
+        log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
+        self.__update_operators(alpha_new)
+        return False
+
+    def __iter__(self):
+        self.iteration = 0
+        while True:
+            self.step()
+            yield self
+
+    def solve(self, callback: Callable = lambda _: _.iteration < 1e4 and _.residue > 1e-4, rtol: float = 0,
+              maxiter=math.inf):
+        if callback is None:
+            def callback(_):
+                return _.iteration < maxiter and _.residue > rtol
+        for sol in self:
+            if not callback(sol):
+                break
+        return self
+
+
+def _eigvals(m: torch.Tensor) -> torch.Tensor:
+    from .backend.hip import hermitian3_eigvals
+    return hermitian3_eigvals(m)

@@ -1,0 +1,363 @@
+"""
+CPU oracle for the modified-Born-series iteration of MacroMax (complex128 NumPy restatement).
+
+TEST INFRASTRUCTURE ONLY. Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module, and only as the checker / the timed CPU baseline. The product package
+``macromax_amd`` never imports it.
+
+Every function cites the reference file:line it restates (paths relative to the reference repository,
+``python/macromax/...``). The arithmetic follows the reference NumPy backend in complex128:
+
+=====================================  =====================================================================
+oracle function                         reference
+=====================================  =====================================================================
+``k_vectors``                           ``utils/ft/grid.py:426-439`` (Grid.f / Grid.k, ifftshifted fftfreq)
+``linear_bound_extinction``             ``bound.py:159-183`` (AbsorbingBound.extinction), ``:196-215``
+``bound_susceptibility``                ``bound.py:186-193``
+``sigma_ee``                            ``solver.py:392-401`` (calc_DeltaEE_beta2 / calc_sigmaEE)
+``find_alpha_beta``                     ``solver.py:446-467`` (non-magnetic fmin) + ``:591-627`` (bias)
+``chi_base``                            ``solver.py:474-484``
+``normalized_source``                   ``solver.py:702-720``
+``longitudinal_projection_ft``          ``backend/__init__.py:606-649``
+``green_op``                            ``solver.py:564-584`` + ``backend/__init__.py:306-319``
+``chi_mul``                             ``backend/__init__.py:415-444`` (einsum 'ij...,jk...->ik...')
+``BornOracle.step``                     ``solver.py:1064-1098`` (one __iter__ cycle incl. divergence guard)
+``BornOracle.relative_residue``         ``solver.py:1026-1042``
+``stockham_fft``                        model of the GPU kernels' Stockham pass schedule (no reference
+                                        counterpart: the reference calls numpy.fft / torch.fft)
+=====================================  =====================================================================
+
+Pinning (what checks each function against something other than its own formula) is listed in
+``tests/test_oracle.py``. Every function above is pinned there; none is "parity unpinned".
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.constants as const
+import scipy.optimize
+
+__all__ = [
+    'k_vectors', 'linear_bound_extinction', 'bound_susceptibility', 'sigma_ee', 'find_alpha_beta',
+    'increase_bias', 'chi_base', 'normalized_source', 'longitudinal_projection_ft', 'green_op',
+    'chi_mul', 'BornOracle', 'stockham_fft', 'plan_radices', 'mat3_max_eig_hermitian', 'setup_problem',
+]
+
+
+# ----------------------------------------------------------------------------------------------------------
+# Grid and boundary (host-side inputs)
+# ----------------------------------------------------------------------------------------------------------
+def k_vectors(shape, step):
+    """k-space sample vectors, one per axis, in ifftshifted (numpy fftfreq) order.
+
+    Follows ``utils/ft/grid.py:426-439``: ``f = Grid(shape, step=1/extent, origin_at_center=False)`` and
+    ``k = 2 pi f``. ``step`` is the (k0-scaled) sample pitch.
+    """
+    return [2.0 * np.pi * np.fft.fftfreq(int(n), d=float(d)) for n, d in zip(shape, step)]
+
+
+def linear_bound_extinction(shape, step, thickness, max_extinction_coefficient=0.25):
+    """Extinction coefficient kappa of a LinearBound on a plaid grid (``bound.py:159-183, 196-215``).
+
+    ``thickness`` and ``max_extinction_coefficient`` broadcast to ``[ndim, 2]`` (axis, front/back).
+    Returns a broadcastable array (singleton dims where the axis does not vary), as the reference does.
+    """
+    ndim = len(shape)
+    thickness = np.broadcast_to(np.asarray(thickness, dtype=float), (ndim, 2))
+    kappa_max = np.broadcast_to(np.asarray(max_extinction_coefficient, dtype=float), (ndim, 2))
+    kappa = 0.0
+    for axis in range(ndim):
+        rng = (np.arange(shape[axis]) * float(step[axis])).reshape(
+            [-1 if _ == axis else 1 for _ in range(ndim)])
+        for back_side in range(2):
+            t = thickness[axis, back_side] * np.sign(step[axis])
+            if not back_side:
+                depth = (rng.ravel()[0] + t) - rng
+            else:
+                depth = rng - (rng.ravel()[-1] - t)
+            depth = depth * np.sign(step[axis])
+            in_boundary = depth > 0
+            if np.any(in_boundary):
+                rel_depth = in_boundary * depth / t
+                kappa = np.maximum(kappa, kappa_max[axis, back_side] * rel_depth * in_boundary)
+    return kappa
+
+
+def bound_susceptibility(extinction, background_permittivity=1.0):
+    """chi_bound = (n_bg + i kappa)^2 - n_bg^2  (``bound.py:186-193``)."""
+    n = np.lib.scimath.sqrt(background_permittivity)
+    return (n + 1j * np.asarray(extinction)) ** 2 - background_permittivity
+
+
+# ----------------------------------------------------------------------------------------------------------
+# Preconditioner (non-magnetic)
+# ----------------------------------------------------------------------------------------------------------
+def mat3_max_eig_hermitian(m):
+    """Largest |eigenvalue| per voxel of Hermitian matrix fields ``m`` of shape [3,3,...] or [1,1,...].
+
+    The reference uses a Cardano closed form (``backend/__init__.py:670-850``); for Hermitian input its
+    result is the exact eigenvalue set, which ``numpy.linalg.eigvalsh`` gives here.
+    """
+    if m.shape[0] == 1 and m.shape[1] == 1:
+        return np.abs(m[0, 0])
+    mm = np.moveaxis(np.moveaxis(m, 0, -1), 0, -1)  # [..., 3, 3]
+    w = np.linalg.eigvalsh(mm)
+    return np.max(np.abs(w), axis=-1)
+
+
+def sigma_ee(eps, alpha_real, beta):
+    """sigma_EE(alpha_r) = sqrt(max lambda((eps - a b)^H (eps - a b))) / |b|  (``solver.py:392-401``)."""
+    ab = alpha_real * beta
+    eps = np.asarray(eps, dtype=np.complex128)
+    if eps.shape[0] == 1 and eps.shape[1] == 1:
+        delta2 = np.abs(eps - ab) ** 2
+        return float(np.sqrt(np.max(delta2))) / abs(beta)
+    d = eps - ab * np.eye(3).reshape(3, 3, *([1] * (eps.ndim - 2)))
+    dh = np.conj(np.swapaxes(d, 0, 1))
+    delta2 = np.einsum('ij...,jk...->ik...', dh, d)
+    return float(np.max(mat3_max_eig_hermitian(delta2))) ** 0.5 / abs(beta)
+
+
+def increase_bias(alpha, bound_thickness_nonzero, wavenumber, max_kernel_field_residue=0.001):
+    """alpha_i <- max(alpha_i + 0.05, 2 n kappa_min)  (``solver.py:591-627``)."""
+    offset = alpha.imag + 0.05
+    t = np.asarray(bound_thickness_nonzero, dtype=float)
+    t = t[t != 0]
+    if t.size > 0:
+        d_min = np.amin(t)
+        min_kappa = np.log(max_kernel_field_residue) / (-wavenumber * d_min)
+        n_real = np.maximum(0, float(alpha.real) + min_kappa ** 2) ** 0.5
+        min_offset = 2 * n_real * min_kappa
+        if offset < min_offset:
+            offset = min_offset
+    return alpha.real + 1j * offset
+
+
+def find_alpha_beta(eps, mu=1.0, bound_thickness=(), wavenumber=1.0):
+    """Non-magnetic preconditioner constants (``solver.py:446-470``).
+
+    beta = Re(1/mu); alpha_r = fmin(sigma_EE) with simplex [[0],[1]], xtol=ftol=0.01, maxiter=maxfun=100;
+    alpha = alpha_r + i sigma_min; conjugated for gain-only media (not handled: raises); then the bias.
+    """
+    beta = float(np.real(1.0 / mu))
+    alpha_real, min_value = scipy.optimize.fmin(
+        lambda a: sigma_ee(eps, float(np.asarray(a).ravel()[0]), beta), 0.0,
+        initial_simplex=[[0.0], [1.0]], disp=False, full_output=True,
+        ftol=0.01, xtol=0.01, maxiter=100, maxfun=100)[:2]
+    alpha = alpha_real[0] + 1.0j * min_value
+    alpha = increase_bias(alpha, bound_thickness, wavenumber)
+    return complex(alpha), beta
+
+
+def chi_base(eps, alpha, beta):
+    """chi = (i/alpha_i) (eps/beta - alpha I)  (``solver.py:480-484``)."""
+    eps = np.asarray(eps, dtype=np.complex128)
+    chi = eps * (1.0j / alpha.imag / beta)
+    if chi.shape[0] == 1:
+        chi = chi - alpha * 1.0j / alpha.imag
+    else:
+        chi = chi - np.eye(3).reshape(3, 3, *([1] * (chi.ndim - 2))) * (alpha * 1.0j / alpha.imag)
+    return chi
+
+
+def normalized_source(S, alpha, beta):
+    """s = (i / (alpha_i beta)) S, shape [C,1,...]  (``solver.py:702-720``)."""
+    return np.asarray(S, dtype=np.complex128) * (1.0j / alpha.imag / beta)
+
+
+# ----------------------------------------------------------------------------------------------------------
+# The operators of one iteration
+# ----------------------------------------------------------------------------------------------------------
+def _kb(k, axis, ndim):
+    return k.reshape([-1 if _ == axis else 1 for _ in range(ndim)])
+
+
+def longitudinal_projection_ft(F, k):
+    """Pi_L F = k (k . F) / k^2 with Pi_L(k=0) = I on the first ndim components, 0 on the others.
+
+    Follows ``backend/__init__.py:606-649`` (DC stored at ``:625``, division skipped at flat index 0 ``:639``,
+    DC restored ``:645``, components >= ndim zeroed ``:646-647``). ``F`` has shape [C,1,*shape].
+    """
+    shape = F.shape[2:]
+    nd = min(len(shape), len(k))
+    kb = [_kb(k[a], a, len(shape)) for a in range(nd)]
+    proj = np.zeros(shape, dtype=np.complex128)
+    for a in range(nd):
+        proj = proj + kb[a] * F[a, 0]
+    k2 = sum(_ ** 2 for _ in kb)
+    k2 = np.broadcast_to(k2, shape)
+    flat = proj.ravel()  # copy-free view of a fresh array
+    flat[1:] /= k2.ravel()[1:]
+    out = np.zeros((max(F.shape[0], nd), 1, *shape), dtype=np.complex128)
+    zero = (0,) * len(shape)
+    for a in range(nd):
+        out[a, 0] = kb[a] * proj
+        out[a, 0][zero] = F[a, 0][zero]
+    return out
+
+
+def green_op(X, k, alpha, vectorial):
+    """G X = ift(g_op(ft(X))) with g = -i alpha_i / (k^2 - alpha)  (``solver.py:564-584``).
+
+    Vectorial: F <- g (F - Pi_L F) + (i alpha_i / alpha) Pi_L F (``solver.py:570-577``).
+    FFT over the spatial axes 2.. (``backend/numpy.py:90-94``), unnormalised forward, 1/N inverse.
+    """
+    axes = tuple(range(2, X.ndim))
+    F = np.fft.fftn(X, axes=axes)
+    shape = X.shape[2:]
+    k2 = sum(_kb(k[a], a, len(shape)) ** 2 for a in range(len(shape)))
+    g = -1.0j * alpha.imag / (k2 - alpha)
+    if vectorial:
+        PL = longitudinal_projection_ft(F, k)
+        F = g * (F - PL) + PL * (1.0j * alpha.imag / alpha)
+    else:
+        F = g * F
+    return np.fft.ifftn(F, axes=axes)
+
+
+def chi_mul(chi, X):
+    """Per-voxel chi X: scalar broadcast if chi is [1,1,...], else 3x3 mat-vec (``backend/__init__.py:415-444``)."""
+    if chi.shape[0] == 1 and chi.shape[1] == 1:
+        return chi * X
+    return np.einsum('ij...,jk...->ik...', chi, X)
+
+
+class BornOracle:
+    """The iteration of ``solver.py:1049-1098`` on complex128 arrays, with the divergence guard.
+
+    State: E (k0^2-scaled field, [C,1,*shape]), chi (base, current alpha), s (normalised source), alpha, beta.
+    ``step()`` performs one cycle and returns True if the update was accepted.
+    """
+
+    def __init__(self, chi, s, alpha, beta, k, vectorial, E0=None):
+        self.chi = np.asarray(chi, dtype=np.complex128)
+        self.s = np.asarray(s, dtype=np.complex128)
+        self.alpha = complex(alpha)
+        self.beta = float(beta)
+        self.k = [np.asarray(_, dtype=float) for _ in k]
+        self.vectorial = bool(vectorial)
+        self.E = np.zeros(self.s.shape, dtype=np.complex128) if E0 is None else np.array(E0, dtype=np.complex128)
+        self.previous_update_norm = np.inf
+        self.iteration = 0
+        self.last_update_norm = None
+        self.accepted = []
+
+    def update_operators(self, alpha_new):
+        """``solver.py:496-529``: rescale s and chi for a new alpha."""
+        a_old = self.alpha
+        self.s = self.s * (a_old.imag / alpha_new.imag)
+        chi = self.chi / (1.0j / a_old.imag)
+        if chi.shape[0] == 1:
+            chi = chi + (a_old - alpha_new)
+        else:
+            chi = chi + np.eye(3).reshape(3, 3, *([1] * (chi.ndim - 2))) * (a_old - alpha_new)
+        self.chi = chi * (1.0j / alpha_new.imag)
+        self.alpha = alpha_new
+
+    def update(self):
+        """d = chi [G (chi E + s) - E]  (``solver.py:1070-1075``)."""
+        d = chi_mul(self.chi, self.E) + self.s
+        d = green_op(d, self.k, self.alpha, self.vectorial)
+        d = d - self.E
+        return chi_mul(self.chi, d)
+
+    def step(self):
+        self.iteration += 1
+        d = self.update()
+        current = float(np.linalg.norm(d.ravel()))
+        self.last_update_norm = current
+        with np.errstate(divide='ignore', invalid='ignore'):
+            rel = current / self.previous_update_norm
+        if rel < 1:
+            self.E = self.E + d
+            self.previous_update_norm = current
+            self.accepted.append(True)
+            return True
+        self.update_operators(self.alpha.real + 1.50j * self.alpha.imag)
+        self.accepted.append(False)
+        return False
+
+    @property
+    def relative_residue(self):
+        """``solver.py:1026-1042``: ||d_prev|| / ||E|| (0 if ||d_prev|| == 0)."""
+        if self.previous_update_norm > 0:
+            return self.previous_update_norm / float(np.linalg.norm(self.E.ravel()))
+        return 0.0
+
+
+def setup_problem(shape, pitch, wavenumber, eps, current_density=None, source_distribution=None,
+                  bound_thickness=None, bound_kappa=0.25, vectorial=True, mu=1.0):
+    """Build (chi, s, alpha, beta, k, vectorial) from physical inputs as ``Solution.__init__`` does.
+
+    ``solver.py:174-185`` (S = -i w mu0 J), ``:212`` (grid scaled by k0), ``:242-247`` (eps + bound chi),
+    ``:270-493`` (preconditioner), ``:702-720`` (source normalisation).
+    ``eps`` is [1,1,*shape] or [3,3,*shape] (broadcastable), ``current_density``/``source_distribution`` [C,*shape].
+    """
+    ndim = len(shape)
+    pitch = np.broadcast_to(np.asarray(pitch, dtype=float), (ndim,))
+    if source_distribution is None:
+        omega = wavenumber * const.c
+        source_distribution = np.asarray(current_density) * (-1j * omega * const.mu_0)
+    S = np.asarray(source_distribution, dtype=np.complex128)
+    C = 3 if vectorial else 1
+    S = np.broadcast_to(S.reshape(C, 1, *shape), (C, 1, *shape))
+    eps = np.asarray(eps, dtype=np.complex128)
+    while eps.ndim < ndim + 2:
+        eps = eps[np.newaxis]
+    thickness = np.zeros((ndim, 2))
+    if bound_thickness is not None:
+        thickness = np.broadcast_to(np.asarray(bound_thickness, dtype=float), (ndim, 2))
+        kappa = linear_bound_extinction(shape, pitch, thickness, bound_kappa)
+        chi_b = bound_susceptibility(kappa)
+        if eps.shape[0] > 1:
+            chi_b = np.eye(3).reshape(3, 3, *([1] * ndim)) * chi_b
+        eps = eps + chi_b
+    alpha, beta = find_alpha_beta(eps, mu, thickness.ravel(), wavenumber)
+    chi = chi_base(np.broadcast_to(eps, (eps.shape[0], eps.shape[1], *shape)), alpha, beta)
+    s = normalized_source(S, alpha, beta)
+    k = k_vectors(shape, pitch * wavenumber)
+    return chi, s, alpha, beta, k
+
+
+# ----------------------------------------------------------------------------------------------------------
+# Model of the GPU FFT schedule (checks the kernels' index arithmetic on the CPU)
+# ----------------------------------------------------------------------------------------------------------
+def plan_radices(n, preferred=(8, 4, 2, 3, 5, 7)):
+    """Factorisation used by the GPU plan: greedily the largest supported radices first (8, then 4, ...)."""
+    radices = []
+    m = int(n)
+    for r in preferred:
+        while m % r == 0 and m > 1:
+            radices.append(r)
+            m //= r
+    if m != 1:
+        raise ValueError(f'FFT length {n} has an unsupported prime factor {m}.')
+    return radices
+
+
+def stockham_fft(x, radices=None, inverse=False):
+    """Self-sorting Stockham FFT along the last axis, pass by pass exactly as the HIP kernels index it.
+
+    Pass with radix R after the radices with product p: for butterfly i in [0, N/R), k = i mod p,
+    inputs x[i + j N/R] (j < R) times twiddle W_N^(j k N/(pR)), an R-point DFT, outputs written to
+    (i - k) R + k + j p. Unnormalised in both directions (the 1/N of the inverse is applied by the caller).
+    """
+    x = np.array(x, dtype=np.complex128)
+    n = x.shape[-1]
+    if radices is None:
+        radices = plan_radices(n)
+    sign = 1.0 if inverse else -1.0
+    tw = np.exp(sign * 2j * np.pi * np.arange(n) / n)
+    p = 1
+    for r in radices:
+        y = np.empty_like(x)
+        m = n // r
+        jj = np.arange(r)
+        dft = np.exp(sign * 2j * np.pi * np.outer(jj, jj) / r)
+        for i in range(m):
+            k = i % p
+            v = x[..., i + jj * m] * tw[(jj * k * (n // (p * r))) % n]
+            out = (i - k) * r + k
+            y[..., out + jj * p] = v @ dft.T
+        x = y
+        p *= r
+    return x

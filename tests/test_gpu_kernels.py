@@ -1,0 +1,171 @@
+"""Parity of the HIP kernels (through the C ABI) with the oracle and numpy.fft, on the GPU.
+
+Tolerances (complex64 kernels against complex128 references):
+- FFT: relative L2 error <= 2e-6 (single precision, O(log N) rounding);
+- one Born iteration (field update d, FP64 norms): relative L2 <= 1e-5 (the per-iteration bar);
+- complex128 kernels: relative L2 <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from oracle import born_oracle as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
+
+
+def _plan(shape, step, dtype, device):
+    from macromax_amd.backend.hip import Plan
+    return Plan(shape, step, dtype, device)
+
+
+FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96), (56, 40), (96, 96, 96),
+              (128, 128, 128), (1, 64, 1), (16, 1, 48), (3, 5, 7), (256, 256), (24, 2, 40)]
+
+
+@pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
+@pytest.mark.parametrize('dtype', ['c64', 'c128'])
+def test_fft_matches_numpy(shape, dtype, device):
+    import torch
+    print(f'fft {shape} {dtype}', flush=True)
+    rng = np.random.default_rng(sum(shape))
+    nb = 3
+    x = rng.standard_normal((nb, *shape)) + 1j * rng.standard_normal((nb, *shape))
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    tol = 2e-6 if dtype == 'c64' else 1e-12
+    plan = _plan(shape, [1.0] * len(shape), tdt, device)
+    xt = torch.as_tensor(x, dtype=tdt, device=device).contiguous()
+    out = torch.empty_like(xt)
+    plan.fft(xt, out, -1)
+    axes = tuple(range(1, 1 + len(shape)))
+    assert rel_l2(out.cpu().numpy(), np.fft.fftn(x, axes=axes)) < tol
+    back = torch.empty_like(xt)
+    plan.fft(out, back, +1)
+    assert rel_l2(back.cpu().numpy(), x) < tol
+    # in place inverse
+    plan.fft(out, out, +1)
+    assert rel_l2(out.cpu().numpy(), x) < tol
+
+
+def _born_case(shape, C, kind, nbatch=1, seed=0, chi_shared=True):
+    rng = np.random.default_rng(seed)
+    V = int(np.prod(shape))
+
+    def crand(*s):
+        return rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    E = 0.3 * crand(nbatch, C, 1, *shape)
+    s = 0.2 * crand(nbatch, C, 1, *shape)
+    nch = 1 if chi_shared else nbatch
+    if kind == 1:
+        chi = 0.3 * crand(nch, 1, 1, *shape)
+    else:
+        chi = 0.2 * crand(nch, 3, 3, *shape)
+    alphas = [complex(1.1 + 0.2 * b, 0.35 + 0.05 * b) for b in range(nbatch)]
+    a = [1.0 - 0.1 * b for b in range(nbatch)]
+    c = [complex(0.05 * b, -0.02 * b) for b in range(nbatch)]
+    return E, s, chi, alphas, a, c
+
+
+def _oracle_update(E, s, chi, alpha, a, c, k, vectorial):
+    eye = np.eye(chi.shape[0]).reshape(chi.shape[0], chi.shape[1], *([1] * (chi.ndim - 2)))
+    chi_eff = a * chi + c * eye
+    o = ob.BornOracle(chi_eff, a * s, alpha, 1.0, k, vectorial, E0=E)
+    d = o.update()
+    return d
+
+
+BORN_CASES = [
+    ((1024,), 1, 1), ((1024,), 3, 1), ((1024,), 3, 2), ((2048,), 3, 2),
+    ((256, 256), 1, 1), ((64, 96), 3, 1), ((48, 40), 3, 2), ((1, 64), 3, 2),
+    ((32, 32, 32), 1, 1), ((24, 32, 40), 3, 1), ((16, 20, 24), 3, 2), ((5, 10, 20), 3, 2),
+    ((14, 21, 8), 3, 1), ((8, 1, 12), 3, 2), ((64, 64, 64), 3, 2),
+]
+
+
+@pytest.mark.parametrize('shape,C,kind', BORN_CASES, ids=lambda v: str(v))
+@pytest.mark.parametrize('dtype', ['c64', 'c128'])
+def test_born_step_matches_oracle(shape, C, kind, dtype, device):
+    import torch
+    if dtype == 'c128' and np.prod(shape) > 40000:
+        pytest.skip('complex128 covered on the smaller grids')
+    print(f'born {shape} C={C} chi_kind={kind} {dtype}', flush=True)
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, seed=len(shape) * 7 + C)
+    step = [0.37 + 0.05 * i for i in range(len(shape))]
+    k = ob.k_vectors(shape, step)
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    rdt = torch.float32 if dtype == 'c64' else torch.float64
+    tol = 1e-5 if dtype == 'c64' else 1e-11
+    plan = _plan(shape, step, tdt, device)
+    Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
+    Eo = torch.zeros_like(Et)
+    st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(chi[0], dtype=tdt, device=device).contiguous()
+    coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
+    norms = torch.zeros(2, dtype=torch.float64, device=device)
+    plan.born_step(Et, Eo, st, ct, C, kind, coef, 1, 0, 0, norms)
+    torch.cuda.synchronize()
+    d_ref = _oracle_update(E[0], s[0], chi[0], alphas[0], a[0], c[0], k, C == 3)
+    d_gpu = Eo.cpu().numpy() - E[0]
+    assert rel_l2(d_gpu, d_ref) < tol
+    dd, ee = norms.cpu().numpy()
+    assert abs(dd - np.sum(np.abs(d_ref) ** 2)) / np.sum(np.abs(d_ref) ** 2) < 2 * tol
+    en = E[0] + d_ref
+    assert abs(ee - np.sum(np.abs(en) ** 2)) / np.sum(np.abs(en) ** 2) < 2 * tol
+    # the input field is untouched
+    assert np.array_equal(Et.cpu().numpy(), E[0].astype(Et.cpu().numpy().dtype))
+
+
+@pytest.mark.parametrize('shape,C,kind,shared', [((64, 48), 1, 1, True), ((12, 16, 20), 3, 2, True),
+                                                 ((12, 16, 20), 3, 1, False), ((512,), 3, 2, False)],
+                         ids=lambda v: str(v))
+def test_born_step_batched(shape, C, kind, shared, device):
+    import torch
+    print(f'born batched {shape} C={C} kind={kind} shared={shared}', flush=True)
+    nb = 3
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, nbatch=nb, seed=11, chi_shared=shared)
+    step = [0.5] * len(shape)
+    k = ob.k_vectors(shape, step)
+    plan = _plan(shape, step, torch.complex64, device)
+    Et = torch.as_tensor(E, dtype=torch.complex64, device=device).contiguous()
+    Eo = torch.zeros_like(Et)
+    st = torch.as_tensor(s, dtype=torch.complex64, device=device).contiguous()
+    ct = torch.as_tensor(chi, dtype=torch.complex64, device=device).contiguous()
+    coef = torch.tensor([[al.real, al.imag, aa, cc.real, cc.imag, 0.0] for al, aa, cc in zip(alphas, a, c)],
+                        dtype=torch.float32, device=device)
+    norms = torch.zeros(2 * nb, dtype=torch.float64, device=device)
+    V = int(np.prod(shape))
+    chi_size = (1 if kind == 1 else 9) * V
+    plan.born_step(Et, Eo, st, ct, C, kind, coef, nb, C * V, 0 if shared else chi_size, norms)
+    torch.cuda.synchronize()
+    out = Eo.cpu().numpy()
+    nrm = norms.cpu().numpy()
+    for b in range(nb):
+        d_ref = _oracle_update(E[b], s[b], chi[0 if shared else b], alphas[b], a[b], c[b], k, C == 3)
+        assert rel_l2(out[b] - E[b], d_ref) < 1e-5
+        assert abs(nrm[2 * b] - np.sum(np.abs(d_ref) ** 2)) / np.sum(np.abs(d_ref) ** 2) < 2e-5
+
+
+@pytest.mark.parametrize('shape', [(5, 10, 20), (32, 24, 16), (40, 48), (64,)], ids=lambda s: 'x'.join(map(str, s)))
+def test_longitudinal_projection_and_chi_mul(shape, device):
+    import torch
+    print(f'lproj/chi_mul {shape}', flush=True)
+    rng = np.random.default_rng(3)
+    F = rng.standard_normal((3, 1, *shape)) + 1j * rng.standard_normal((3, 1, *shape))
+    k = ob.k_vectors(shape, [1.0] * len(shape))
+    plan = _plan(shape, [1.0] * len(shape), torch.complex64, device)
+    Ft = torch.as_tensor(F, dtype=torch.complex64, device=device).contiguous()
+    out = torch.empty_like(Ft)
+    plan.longitudinal_projection(Ft, out)
+    assert rel_l2(out.cpu().numpy(), ob.longitudinal_projection_ft(F, k)) < 1e-6
+    chi = rng.standard_normal((3, 3, *shape)) + 1j * rng.standard_normal((3, 3, *shape))
+    ct = torch.as_tensor(chi, dtype=torch.complex64, device=device).contiguous()
+    y = torch.empty_like(Ft)
+    plan.chi_mul(ct, 2, Ft, y, 3, 1)
+    assert rel_l2(y.cpu().numpy(), ob.chi_mul(chi, F)) < 1e-6
+    x = torch.empty(1, dtype=torch.float64, device=device)
+    plan.norm2(Ft, x)
+    F32 = F.astype(np.complex64).astype(np.complex128)  # the values the device holds, summed in float64
+    assert abs(float(x.item()) - np.sum(np.abs(F32) ** 2)) / np.sum(np.abs(F32) ** 2) < 1e-12

@@ -1,0 +1,295 @@
+"""Pins of the CPU oracle (oracle/born_oracle.py) against the reference's own known answers and mathematics.
+
+Each oracle function is checked against something other than its own formula:
+- k_vectors: Grid.f / Grid.k values of the reference tests (tests/test_grid.py:237-251) and k2 at [1,1,1]
+  (tests/test_backend.py:301-309);
+- linear_bound_extinction / bound_susceptibility: kappa tables of tests/test_bound.py:10-75;
+- longitudinal_projection_ft: the known cases of tests/test_backend.py:251-299 plus projector invariants;
+- stockham_fft (the GPU pass schedule): brute-force DFT and numpy.fft; a planted twiddle-sign mistake fails;
+- green_op: single Fourier modes land on the right k bin with the 1/N of the inverse transform;
+- chi_mul: per-voxel matrix products written out by hand;
+- find_alpha_beta / increase_bias: brute-force scan of sigma_EE and the closed-form bias;
+- mat3_max_eig_hermitian: numpy's general (non-Hermitian) eigen-solver;
+- BornOracle (iteration, accept/reject, residue): the analytic plane-wave solutions of tests/test_solution.py:91-358
+  (iteration counts <= 70 / <= 85 to residue 1e-6, decimal=3 field error), the bookkeeping definitions of
+  tests/test_solution.py:59-77, and the divergence guard on an injected under-damped alpha.
+"""
+import numpy as np
+import numpy.testing as npt
+import pytest
+import scipy.constants as const
+
+from oracle import born_oracle as ob
+
+
+def test_k_vectors_match_reference_grid_tests():
+    npt.assert_allclose(ob.k_vectors((5,), (1.0,))[0], np.array([0, 1 / 5, 2 / 5, -2 / 5, -1 / 5]) * 2 * np.pi)
+    npt.assert_allclose(ob.k_vectors((5,), (2.0,))[0], np.array([0, 1 / 10, 2 / 10, -2 / 10, -1 / 10]) * 2 * np.pi)
+    # tests/test_backend.py:301-309 on the 5x10x20 grid, pitch 100 nm, k0 = 1e7 (k0-scaled pitch 1)
+    shape = np.array([5, 10, 20])
+    k = ob.k_vectors(shape, (1.0, 1.0, 1.0))
+    k2_111 = k[0][1] ** 2 + k[1][1] ** 2 + k[2][1] ** 2
+    npt.assert_almost_equal(k2_111 / (2 * np.pi) ** 2, np.sum(1.0 / shape ** 2))
+
+
+def test_linear_bound_matches_reference_tables():
+    kappa = ob.linear_bound_extinction((10,), (1e-6,), 4e-6, 0.1)
+    npt.assert_array_almost_equal(kappa, [0.100, 0.075, 0.050, 0.025, 0.0, 0.0, 0.025, 0.050, 0.075, 0.100])
+    kappa = ob.linear_bound_extinction((10,), (1e-6,), [4e-6, 2e-6], 0.1)
+    npt.assert_array_almost_equal(kappa, [0.100, 0.075, 0.050, 0.025, 0.0, 0.0, 0.0, 0.0, 0.050, 0.100])
+    kappa = ob.linear_bound_extinction((10,), (1e-6,), 4e-6, [0.1, 0.2])
+    npt.assert_array_almost_equal(kappa, [0.100, 0.075, 0.050, 0.025, 0.0, 0.0, 0.050, 0.100, 0.150, 0.200])
+    kappa = ob.linear_bound_extinction((2, 10), (1e-6, 1e-6), [[0, 0], [4e-6, 2e-6]], [0.1, 0.1])
+    npt.assert_array_almost_equal(kappa, [[0.100, 0.075, 0.050, 0.025, 0.0, 0.0, 0.0, 0.0, 0.050, 0.100]])
+    n_bg = 1.33
+    chi = ob.bound_susceptibility(np.array([0.0, 0.1]), n_bg ** 2)
+    npt.assert_array_almost_equal(chi, (n_bg + 1j * np.array([0.0, 0.1])) ** 2 - n_bg ** 2)
+
+
+@pytest.mark.parametrize('idx,val,expected', [
+    ((0, 0, 10), [2, 3, 4], [0, 0, 4]),
+    ((0, 0, 2), [2, 3, 4], [0, 0, 4]),
+    ((0, 5, 10), [4, 5, 6], [0, 5.5, 5.5]),
+])
+def test_longitudinal_projection_reference_cases(idx, val, expected):
+    """tests/test_backend.py:276-299 (5x10x20 grid, pitch 1)."""
+    k = ob.k_vectors((5, 10, 20), (1, 1, 1))
+    A = np.zeros((3, 1, 5, 10, 20), complex)
+    A[(slice(None), 0) + idx] = val
+    L = ob.longitudinal_projection_ft(A, k)
+    C = np.zeros_like(A)
+    C[(slice(None), 0) + idx] = expected
+    npt.assert_array_almost_equal(L, C)
+
+
+@pytest.mark.parametrize('idx,val,expected', [
+    ((0, 0, 10), [2, 3, 4], [2, 3, 0]),
+    ((0, 0, 3), [2, 3, 4], [2, 3, 0]),
+    ((0, 5, 10), [0, 3, 3], [0, 0, 0]),
+])
+def test_transversal_projection_reference_cases(idx, val, expected):
+    """tests/test_backend.py:251-274: Pi_T = I - Pi_L."""
+    k = ob.k_vectors((5, 10, 20), (1, 1, 1))
+    A = np.zeros((3, 1, 5, 10, 20), complex)
+    A[(slice(None), 0) + idx] = val
+    T = A - ob.longitudinal_projection_ft(A, k)
+    C = np.zeros_like(A)
+    C[(slice(None), 0) + idx] = expected
+    npt.assert_array_almost_equal(T, C)
+
+
+def test_longitudinal_projection_invariants():
+    rng = np.random.default_rng(1)
+    shape = (6, 7, 8)
+    k = ob.k_vectors(shape, (0.7, 0.9, 1.1))
+    F = rng.standard_normal((3, 1, *shape)) + 1j * rng.standard_normal((3, 1, *shape))
+    L = ob.longitudinal_projection_ft(F, k)
+    npt.assert_allclose(ob.longitudinal_projection_ft(L, k), L, atol=1e-12)      # idempotent
+    T = F - L
+    npt.assert_allclose(ob.longitudinal_projection_ft(T, k), 0 * T, atol=1e-12)   # Pi_L Pi_T = 0
+    kk = np.stack(np.broadcast_arrays(k[0][:, None, None], k[1][None, :, None], k[2][None, None, :]))
+    npt.assert_allclose(np.sum(kk * T[:, 0], axis=0).ravel()[1:], 0, atol=1e-10)  # T is transverse
+    npt.assert_allclose(L[:, 0, 0, 0, 0], F[:, 0, 0, 0, 0])                       # DC is longitudinal
+    # 2D vectorial: the third component has no longitudinal part, not even at DC
+    k2d = ob.k_vectors((6, 7), (1.0, 1.0))
+    F2 = rng.standard_normal((3, 1, 6, 7)) + 0j
+    L2 = ob.longitudinal_projection_ft(F2, k2d)
+    npt.assert_array_equal(L2[2], 0)
+    npt.assert_allclose(L2[:2, 0, 0, 0], F2[:2, 0, 0, 0])
+
+
+def _dft(x, inverse=False):
+    n = x.shape[-1]
+    sign = 1 if inverse else -1
+    W = np.exp(sign * 2j * np.pi * np.outer(np.arange(n), np.arange(n)) / n)
+    return x @ W.T
+
+
+@pytest.mark.parametrize('n', [2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 35, 40, 49, 56, 64, 96])
+def test_stockham_schedule_matches_brute_force_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    npt.assert_allclose(ob.stockham_fft(x), _dft(x), atol=1e-10 * n)
+    npt.assert_allclose(ob.stockham_fft(x, inverse=True), _dft(x, inverse=True), atol=1e-10 * n)
+
+
+def test_stockham_schedule_large_sizes_and_planted_mistake():
+    rng = np.random.default_rng(0)
+    for n in (256, 384, 1000, 1024):
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        npt.assert_allclose(ob.stockham_fft(x), np.fft.fft(x), rtol=0, atol=1e-9 * np.sqrt(n))
+    x = rng.standard_normal(1001) + 1j * rng.standard_normal(1001)  # radices 7, 11, 13 of the schedule model
+    npt.assert_allclose(ob.stockham_fft(x, radices=[7, 11, 13]), np.fft.fft(x), rtol=0, atol=1e-9 * np.sqrt(1001))
+    # a planted mistake in the schedule (twiddles of the wrong sign) must be detected by the same check
+    x = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    wrong = ob.stockham_fft(x, inverse=True)  # conjugate twiddles and butterflies
+    assert np.max(np.abs(wrong - np.fft.fft(x))) > 1.0
+
+
+def test_plan_radices():
+    assert ob.plan_radices(256) == [8, 8, 4]
+    assert ob.plan_radices(96) == [8, 4, 3]
+    assert ob.plan_radices(20) == [4, 5]
+    with pytest.raises(ValueError):
+        ob.plan_radices(11)
+
+
+def test_green_op_single_mode_lands_on_its_k_bin():
+    """A plane wave exp(i k_m x) is an eigenfunction of G with eigenvalue g(k_m) (scalar case)."""
+    shape = (8, 12)
+    step = (0.6, 0.45)
+    k = ob.k_vectors(shape, step)
+    alpha = complex(1.3, 0.4)
+    x0 = np.arange(shape[0])[:, None] * step[0]
+    x1 = np.arange(shape[1])[None, :] * step[1]
+    for m in [(0, 0), (1, 0), (3, 5), (7, 11), (4, 6)]:
+        km = (2 * np.pi * np.fft.fftfreq(shape[0], step[0])[m[0]], 2 * np.pi * np.fft.fftfreq(shape[1], step[1])[m[1]])
+        X = np.exp(1j * (km[0] * x0 + km[1] * x1))[None, None]
+        GX = ob.green_op(X, k, alpha, False)
+        g = -1j * alpha.imag / (km[0] ** 2 + km[1] ** 2 - alpha)
+        npt.assert_allclose(GX, g * X, atol=1e-12)
+
+
+def test_green_op_vectorial_mode_splits_transverse_longitudinal():
+    shape = (8, 10, 6)
+    step = (0.5, 0.5, 0.5)
+    k = ob.k_vectors(shape, step)
+    alpha = complex(1.1, 0.3)
+    m = (2, 3, 1)
+    km = np.array([k[a][m[a]] for a in range(3)])
+    grids = np.meshgrid(*[np.arange(n) * s for n, s in zip(shape, step)], indexing='ij')
+    phase = np.exp(1j * sum(km[a] * grids[a] for a in range(3)))
+    pol = np.array([1.0, -2.0, 0.5 + 1j])
+    X = pol[:, None, None, None, None] * phase[None, None]
+    GX = ob.green_op(X, k, alpha, True)
+    khat = km / np.linalg.norm(km)
+    pl = khat * (khat @ pol)
+    pt = pol - pl
+    g = -1j * alpha.imag / (km @ km - alpha)
+    expected = (g * pt + pl * (1j * alpha.imag / alpha))[:, None, None, None, None] * phase[None, None]
+    npt.assert_allclose(GX, expected, atol=1e-12)
+
+
+def test_chi_mul_by_hand():
+    rng = np.random.default_rng(2)
+    chi = rng.standard_normal((3, 3, 2, 3)) + 1j * rng.standard_normal((3, 3, 2, 3))
+    X = rng.standard_normal((3, 1, 2, 3)) + 1j * rng.standard_normal((3, 1, 2, 3))
+    Y = ob.chi_mul(chi, X)
+    for i in range(2):
+        for j in range(3):
+            npt.assert_allclose(Y[:, 0, i, j], chi[:, :, i, j] @ X[:, 0, i, j])
+    chi1 = rng.standard_normal((1, 1, 2, 3)) + 0j
+    npt.assert_allclose(ob.chi_mul(chi1, X), chi1[0, 0] * X)
+
+
+def test_mat3_max_eig_against_general_solver():
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((3, 3, 5)) + 1j * rng.standard_normal((3, 3, 5))
+    H = A + np.conj(np.swapaxes(A, 0, 1))
+    got = ob.mat3_max_eig_hermitian(H)
+    for v in range(5):
+        npt.assert_allclose(got[v], np.max(np.abs(np.linalg.eig(H[:, :, v])[0])), rtol=1e-10)
+
+
+def test_alpha_against_brute_force_scan_and_bias():
+    rng = np.random.default_rng(5)
+    eps = (1.0 + 1.5 * rng.random((1, 1, 40)) + 0.2j * rng.random((1, 1, 40)))
+    alpha, beta = ob.find_alpha_beta(eps)
+    scan = np.linspace(-1, 4, 50001)
+    sig = np.array([np.max(np.abs(eps - a)) for a in scan])
+    a_best = scan[np.argmin(sig)]
+    assert abs(alpha.real - a_best) < 0.02
+    assert abs((alpha.imag - 0.05) - sig.min()) < 0.01 * sig.min() + 1e-3  # alpha_i = sigma_min + 0.05 (periodic)
+    # with a bound: alpha_i >= 2 n kappa_min with kappa_min = ln(1e3) / (k0 d_min)
+    a2 = ob.increase_bias(complex(1.0, 0.1), [2.0, 0.0], wavenumber=1.0)
+    kmin = np.log(1e3) / 2.0
+    npt.assert_allclose(a2.imag, 2 * np.sqrt(1.0 + kmin ** 2) * kmin)
+    a3 = ob.increase_bias(complex(1.0, 5.0), [20.0], wavenumber=1.0)  # thick bound: the +0.05 margin wins
+    npt.assert_allclose(a3.imag, 5.05)
+
+
+def _plane_wave_problem(kind):
+    wavelength = 500e-9
+    k0 = 2 * np.pi / wavelength
+    n = 1024
+    pitch = wavelength / 16
+    bt = 10e-6
+    x = pitch * np.arange(n) - bt
+    f = np.maximum(0, np.maximum((x[0] + bt) - x, x - (x[-1] - bt)) / bt)
+    eps = (1 + 1j * 0.1 * f) ** 2
+    J = np.zeros(n)
+    J[np.argmin(np.abs(x))] = 1.0
+    if kind == 'scalar':
+        epsm, Jc, C = eps[None, None], J[None], 1
+    else:
+        Jc, C = np.array([0, 1, 0])[:, None] * J[None], 3
+        if kind == 'anisotropic':
+            epsm = eps * np.eye(3)[..., None]
+            epsm[:, :, f <= 0] = np.array([[1.1, 0, 0.3], [0, 1, 0], [0.3, 0, 1.1]])[..., None]
+        else:
+            epsm = eps[None, None]
+    chi, s, alpha, beta, k = ob.setup_problem((n,), pitch, k0, epsm, current_density=Jc.astype(np.complex64),
+                                             vectorial=(C == 3))
+    ref = const.mu_0 * pitch / 2 * const.c * np.exp(1j * k0 * np.abs(x)) * np.exp(-0.1 * k0 * bt * f ** 2 / 2)
+    if C == 3:
+        ref = np.array([0, 1, 0])[:, None] * ref
+    else:
+        ref = ref[None]
+    sel = (wavelength * 10 < x) & (x < x[-1] - bt - wavelength * 10)
+    return chi, s, alpha, beta, k, C, k0, ref, sel
+
+
+@pytest.mark.parametrize('kind,max_iter', [('scalar', 70), ('vectorial', 70), ('anisotropic', 85)])
+def test_oracle_reproduces_analytic_plane_wave(kind, max_iter):
+    """tests/test_solution.py:91-358: residue < 1e-6 within 70 (85) iterations, field error decimal=3."""
+    chi, s, alpha, beta, k, C, k0, ref, sel = _plane_wave_problem(kind)
+    o = ob.BornOracle(chi, s, alpha, beta, k, C == 3)
+    while True:
+        o.step()
+        if not (o.relative_residue > 1e-6 and o.iteration < 1e4):
+            break
+    assert o.relative_residue < 1e-6
+    assert o.iteration <= max_iter
+    E = o.E[:, 0] / k0 ** 2
+    err = E - ref
+    npt.assert_almost_equal(np.sqrt(np.mean(np.abs(err[:, sel]) ** 2)) / np.sqrt(np.mean(np.abs(E[:, sel]) ** 2)), 0, decimal=3)
+    npt.assert_almost_equal(np.sqrt(np.mean(np.abs(err) ** 2)) / np.sqrt(np.mean(np.abs(E) ** 2)), 0, decimal=2)
+    if C == 3:
+        npt.assert_array_equal(np.abs(E[[0, 2]]) < 1e-30, True) if kind == 'vectorial' else None
+
+
+def test_oracle_bookkeeping_definitions():
+    """tests/test_solution.py:59-77: previous_update_norm = ||E1 - E0||, residue = ||E1 - E0|| / ||E1||."""
+    chi, s, alpha, beta, k, C, k0, ref, sel = _plane_wave_problem('scalar')
+    o = ob.BornOracle(chi, s, alpha, beta, k, False)
+    E0 = o.E.copy()
+    o.step()
+    E1 = o.E.copy()
+    npt.assert_allclose(o.previous_update_norm, np.linalg.norm((E1 - E0).ravel()))
+    npt.assert_allclose(o.relative_residue, np.linalg.norm((E1 - E0).ravel()) / np.linalg.norm(E1.ravel()))
+    o.step()
+    E2 = o.E.copy()
+    npt.assert_allclose(o.relative_residue, np.linalg.norm((E2 - E1).ravel()) / np.linalg.norm(E2.ravel()))
+
+
+def test_oracle_divergence_guard():
+    """solver.py:1083-1096: a growing update is rejected, E kept, alpha_i raised by 1.5 until it converges."""
+    chi, s, alpha, beta, k, C, k0, ref, sel = _plane_wave_problem('scalar')
+    eps = chi / (1.0j / alpha.imag) + alpha  # recover eps (beta = 1)
+    bad_alpha = complex(alpha.real, alpha.imag * 0.05)
+    chi_bad = ob.chi_base(eps, bad_alpha, 1.0)
+    s_bad = s * (alpha.imag / bad_alpha.imag)
+    o = ob.BornOracle(chi_bad, s_bad, bad_alpha, 1.0, k, False)
+    rejected = 0
+    for _ in range(60):
+        E_before = o.E.copy()
+        a_before = o.alpha
+        if not o.step():
+            rejected += 1
+            npt.assert_array_equal(o.E, E_before)
+            npt.assert_allclose(o.alpha.imag, 1.5 * a_before.imag)
+            npt.assert_allclose(o.alpha.real, a_before.real)
+    assert rejected >= 1
+    # rescaling chi and s in place equals rebuilding them from eps for the new alpha
+    npt.assert_allclose(o.chi, ob.chi_base(eps, o.alpha, 1.0), rtol=1e-10, atol=1e-12)
+    npt.assert_allclose(o.s, s * (alpha.imag / o.alpha.imag), rtol=1e-10)
