@@ -21,7 +21,7 @@ extern "C" {
 
 #define MMX_OK 0
 #define MMX_EINVAL -1    /* bad argument (shape, size, kind) */
-#define MMX_EUNSUP -2    /* unsupported FFT length (prime factor > 13 or > 4096 points on one axis) */
+#define MMX_EUNSUP -2    /* unsupported FFT length (prime factor > 7, more than 4096 points, or too large for LDS) */
 #define MMX_EHIP -3      /* HIP runtime error */
 #define MMX_ENOMEM -4
 
@@ -40,6 +40,26 @@ const char* mmx_last_error(void);
  */
 int mmx_plan_create(mmx_plan_t* plan, int ndim, const int64_t* shape, const double* step, int dtype);
 int mmx_plan_destroy(mmx_plan_t plan);
+
+/*
+ * Tuning parameters of a plan (no counterpart in the reference: its FFTs are library calls). 0 restores the
+ * automatic choice. Values out of range are clamped to what the device and the tile shape allow.
+ *   MMX_PARAM_G_AXIS     which non-trivial axis (0 = outermost) hosts the fused forward/Green/inverse pass;
+ *                        the innermost axis always hosts the susceptibility prologue and update epilogue
+ *   MMX_PARAM_P_ROWS     pencils per tile in passes along the contiguous axis
+ *   MMX_PARAM_P_STRIDED  pencils (contiguous columns) per tile in single-component strided passes
+ *   MMX_PARAM_P_GREEN    pencils per tile in the fused Green pass (all components in one tile)
+ *   MMX_PARAM_XCD_REMAP  1: remap workgroups so that neighbouring tiles run on one XCD; 0: identity
+ *   MMX_PARAM_LDS_BUDGET preferred LDS bytes per workgroup
+ */
+#define MMX_PARAM_G_AXIS 1
+#define MMX_PARAM_P_ROWS 2
+#define MMX_PARAM_P_STRIDED 3
+#define MMX_PARAM_P_GREEN 4
+#define MMX_PARAM_XCD_REMAP 5
+#define MMX_PARAM_LDS_BUDGET 6
+int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
+int mmx_plan_get_param(mmx_plan_t plan, int param, int64_t* value);
 
 /*
  * FFT over the spatial axes of `nbatch` consecutive fields of the plan's grid.
@@ -61,7 +81,8 @@ int mmx_fft(mmx_plan_t plan, const void* in, void* out, int64_t nbatch, int dire
  * solver.py:1070-1085):
  *     d = chi_eff [ G(chi_eff E + s_eff) - E ],   Eout = E + d,
  * G the scalar (ncomp == 1) or dyadic (ncomp == 3) Green's operator (solver.py:564-584,
- * backend/__init__.py:606-649). chi_kind 1: chi is [1,1,grid] (isotropic); 2: chi is [3,3,grid] (needs ncomp 3).
+ * backend/__init__.py:606-649). chi_kind 1: chi is [1,1,grid] (isotropic); 2: chi is [3,3,grid] (needs ncomp 3);
+ * 3: chi is complex-symmetric (chi^T = chi, reciprocal media) packed as [6,grid] = {xx, xy, xz, yy, yz, zz}.
  * E, Eout: [nbatch][ncomp][grid]; s: [nbatch or 1][ncomp][grid] (s_bstride 0 shares one source);
  * chi: [nbatch or 1][chi size] (chi_bstride 0 shares one medium). Eout is also the work buffer; E is only read.
  * norms (device, 2*nbatch doubles): {sum |d|^2, sum |Eout|^2} per problem, accumulated in FP64 with a fixed
@@ -76,7 +97,7 @@ int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stre
 
 /*
  * y = chi x per voxel for `nbatch` fields x of ncomp components (BackEnd.mul(chi, x), backend/__init__.py:415-444,
- * torch.py:220-249, for chi of shape [1,1,grid] or [3,3,grid]). y may alias x.
+ * torch.py:220-249, for chi of shape [1,1,grid], [3,3,grid] or packed symmetric [6,grid]). y may alias x.
  */
 int mmx_chi_mul(mmx_plan_t plan, const void* chi, int chi_kind, const void* x, void* y, int ncomp,
                 int64_t nbatch, void* stream);

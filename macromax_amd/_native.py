@@ -16,8 +16,11 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmacromax
 MMX_C64 = 0
 MMX_C128 = 1
 
-EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_fft', 'mmx_born_step',
-            'mmx_norm2', 'mmx_chi_mul', 'mmx_longitudinal_projection_ft')
+EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
+            'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_norm2', 'mmx_chi_mul',
+            'mmx_longitudinal_projection_ft')
+
+PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6}
 
 
 class NativeError(RuntimeError):
@@ -38,6 +41,10 @@ def _declare(L):
     L.mmx_plan_create.argtypes = [ctypes.POINTER(c_p), c_int, ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), c_int]
     L.mmx_plan_destroy.restype = c_int
     L.mmx_plan_destroy.argtypes = [c_p]
+    L.mmx_plan_set_param.restype = c_int
+    L.mmx_plan_set_param.argtypes = [c_p, c_int, c_i64]
+    L.mmx_plan_get_param.restype = c_int
+    L.mmx_plan_get_param.argtypes = [c_p, c_int, ctypes.POINTER(c_i64)]
     L.mmx_fft.restype = c_int
     L.mmx_fft.argtypes = [c_p, c_p, c_p, c_i64, c_int, c_p]
     L.mmx_born_step.restype = c_int

@@ -93,6 +93,15 @@ class Plan:
                 pass
             self.handle = None
 
+    def set_param(self, name: str, value: int):
+        """Set a tuning parameter (see include/macromax_hip.h, MMX_PARAM_*); 0 = automatic."""
+        _native.check(self._lib.mmx_plan_set_param(self.handle, _native.PARAMS[name], int(value)), 'mmx_plan_set_param')
+
+    def get_param(self, name: str) -> int:
+        v = ctypes.c_int64()
+        _native.check(self._lib.mmx_plan_get_param(self.handle, _native.PARAMS[name], ctypes.byref(v)), 'mmx_plan_get_param')
+        return int(v.value)
+
     def _check(self, t, what):
         if not isinstance(t, torch.Tensor):
             raise TypeError(f'{what}: expected a torch.Tensor')
@@ -122,7 +131,7 @@ class Plan:
             raise ValueError('born_step: norms must be a float64 device tensor of 2*nbatch elements')
         if E.numel() != nbatch * ncomp * self.V or Eout.numel() != E.numel():
             raise ValueError('born_step: E / Eout size mismatch')
-        chi_size = (1 if chi_kind == 1 else 9) * self.V
+        chi_size = {1: 1, 2: 9, 3: 6}[int(chi_kind)] * self.V
         if chi.numel() != (chi_size if chi_bstride == 0 else chi_size * nbatch):
             raise ValueError('born_step: chi size mismatch')
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):

@@ -58,6 +58,11 @@ struct mmx_plan_s {
     size_t scratch_bytes;
     int device;
     size_t lds_max;  // device limit of shared memory per block
+    // tuning (0 = automatic)
+    int g_axis;
+    int p_rows, p_strided, p_green;
+    int xcd_remap;
+    size_t lds_budget;
 };
 
 namespace {
@@ -127,7 +132,8 @@ struct TileCfg {
 // fused: tiles hold `ncomp` components of one problem, grid = tiles_per_problem * nbatch.
 // plain: (batch, component, outer) folded into nouter, ncomp = 1, `nbatch` counts fields.
 template <typename T>
-int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_t lds_max, TileCfg& t) {
+int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_t lds_max, TileCfg& t,
+                int p_request = 0, size_t lds_budget = kLdsBudget) {
     int E = sizeof(T) == 4 ? kElemsF : kElemsD;
     t.big = 0;
     if ((int64_t)(fused ? ncomp : 1) * ax.N > (int64_t)1024 * E) {
@@ -142,7 +148,7 @@ int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_
     const int64_t mult = fused ? nbatch : 1;  // independent tile groups besides nouter
     t.pad = rows ? 1 : 0;
     const size_t per = (size_t)ncomp * (size_t)(N + t.pad) * el;
-    size_t budget = kLdsBudget;
+    size_t budget = lds_budget > lds_max ? lds_max : lds_budget;
     if (N * el + per > budget) budget = lds_max;
     if (N * el + per > budget) return fail(MMX_EUNSUP, "FFT length too large for one LDS tile");
     int64_t pm = (int64_t)((budget - N * el) / per);
@@ -154,16 +160,19 @@ int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_
         int64_t target = 8;
         const int64_t by_tiles = total_rows / 1024;  // keep >= ~1024 tiles when the field allows it
         if (by_tiles < target) target = by_tiles < 1 ? 1 : by_tiles;
+        if (p_request > 0) target = p_request;
         if (pm > target) pm = target;
         if (pm > t.nouter) pm = t.nouter;
-        t.P = pow2floor(pm < 1 ? 1 : pm);
+        t.P = p_request > 0 ? (int)(pm < 1 ? 1 : pm) : pow2floor(pm < 1 ? 1 : pm);
         t.ntile_inner = 1;
         t.tiles_per_problem = (t.nouter + t.P - 1) / t.P;
     } else {
-        if (pm > 16) pm = 16;
-        t.P = ax.inner < pm ? (int)ax.inner : pow2floor(pm < 1 ? 1 : pm);
+        const int64_t target = p_request > 0 ? p_request : 16;
+        if (pm > target) pm = target;
+        t.P = ax.inner < pm ? (int)ax.inner : (p_request > 0 ? (int)(pm < 1 ? 1 : pm) : pow2floor(pm < 1 ? 1 : pm));
         // few tiles in total: trade segment length for parallelism
-        while (t.P > 2 && t.nouter * mult * ((ax.inner + t.P - 1) / t.P) < 512) t.P /= 2;
+        if (p_request <= 0)
+            while (t.P > 2 && t.nouter * mult * ((ax.inner + t.P - 1) / t.P) < 512) t.P /= 2;
         t.ntile_inner = (ax.inner + t.P - 1) / t.P;
         t.tiles_per_problem = t.nouter * t.ntile_inner;
     }
@@ -179,8 +188,18 @@ int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_
 template <typename T>
 void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.N = ax.N;
-    a.nrad = ax.nrad;
-    for (int r = 0; r < kMaxRadices; ++r) a.rad[r] = r < ax.nrad ? ax.rad[r] : 1;
+    a.rp.nrad = ax.nrad;
+    int prod = 1;
+    for (int r = 0; r < kMaxRadices; ++r) {
+        const int R = r < ax.nrad ? ax.rad[r] : 1;
+        a.rp.rad[r] = R;
+        a.rp.per[r] = make_fdiv(ax.N / R);
+        a.rp.p[r] = make_fdiv(prod);
+        prod *= R;
+    }
+    a.fN = make_fdiv(ax.N);
+    a.fn1 = make_fdiv64(p->shape3[1]);
+    a.fn2 = make_fdiv64(p->shape3[2]);
     a.inner = ax.inner;
     a.tw = reinterpret_cast<const cx<T>*>(ax.tw);
     a.n0 = p->shape3[0];
@@ -198,11 +217,11 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 template <typename T, int KIND, int E>
 void launch_kind_e(const TileCfg& t, const PassArgs<T>& a, hipStream_t stream) {
     if (t.lds > 65536) {
-        static bool raised = false;  // once per instantiation
-        if (!raised) {
+        static size_t raised = 0;  // per instantiation: the largest dynamic LDS size requested so far
+        if (t.lds > raised) {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&pencil_pass_kernel<T, E, KIND>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)t.lds);
-            raised = true;
+            raised = t.lds;
         }
     }
     hipLaunchKernelGGL((pencil_pass_kernel<T, E, KIND>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, stream, a);
@@ -220,8 +239,11 @@ template <typename T>
 int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
                 hipStream_t stream) {
     TileCfg t;
-    int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t);
+    const bool rows = ax.inner == 1;
+    const int preq = rows ? p->p_rows : (fused ? p->p_green : p->p_strided);
+    int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t, preq, p->lds_budget);
     if (rc) return rc;
+    a.xcd_remap = p->xcd_remap;
     if (!fused) {
         a.ncomp = 1;
         a.bstride = 0;
@@ -231,6 +253,11 @@ int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t 
     a.pad = t.pad;
     a.ntile_inner = (int)t.ntile_inner;
     a.tiles_per_problem = t.tiles_per_problem;
+    a.fP = make_fdiv(t.P);
+    a.fncomp = make_fdiv(a.ncomp > 0 ? a.ncomp : 1);
+    a.fntile = make_fdiv64(t.ntile_inner);
+    a.ftpp = make_fdiv64(t.tiles_per_problem);
+    if ((int64_t)a.ncomp * t.P * (ax.N + t.pad) >= (1 << 22)) return fail(MMX_EUNSUP, "internal: tile index range");
     if (t.nblocks <= 0) return MMX_OK;
     if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
     switch (kind) {
@@ -282,15 +309,15 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
               const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
               hipStream_t stream) {
     if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
-    if (chi_kind != 1 && chi_kind != 2) return fail(MMX_EINVAL, "chi_kind must be 1 or 2");
-    if (chi_kind == 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
+    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
+    if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
     if (nbatch <= 0) return MMX_OK;
     if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
     const int d = p->naxes;
     const AxisPlan& inner = p->ax[d - 1];
     // partial sums: one pair per tile of the last pass
     TileCfg tl;
-    int rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl);
+    int rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl, p->p_rows, p->lds_budget);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
     rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
@@ -325,11 +352,16 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
         rc = launch_pass<T>(p, inner, a, nbatch, true, kAll, stream);
         if (rc) return rc;
     } else {
+        // the Green pass runs on axis g (0 = outermost); the innermost axis carries prologue and epilogue
+        int g = p->g_axis;
+        if (g < 0) g = 0;
+        if (g > d - 2) g = d - 2;
+        int others[3], nothers = 0;
+        for (int q = d - 2; q >= 0; --q)
+            if (q != g) others[nothers++] = q;
         {   // pass 1: prologue + forward FFT along the innermost axis
             PassArgs<T> a = base;
             fill_geometry<T>(p, inner, a);
-            a.prologue = 1;
-            a.dir = -1;
             a.ncomp = C;
             a.bstride = (int64_t)C * p->V;
             a.src = W;
@@ -337,40 +369,35 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
             rc = launch_pass<T>(p, inner, a, nbatch, true, kProFwd, stream);
             if (rc) return rc;
         }
-        for (int q = d - 2; q >= 1; --q) {  // middle axes, forward
+        for (int q = 0; q < nothers; ++q) {  // remaining axes, forward
             PassArgs<T> a = base;
-            fill_geometry<T>(p, p->ax[q], a);
-            a.dir = -1;
+            fill_geometry<T>(p, p->ax[others[q]], a);
             a.src = W;
             a.dst = W;
-            rc = launch_pass<T>(p, p->ax[q], a, nbatch * C, false, kFwd, stream);
+            rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kFwd, stream);
             if (rc) return rc;
         }
-        {   // outermost axis: forward FFT, Green's operator, inverse FFT
+        {   // axis g: forward FFT, Green's operator, inverse FFT
             PassArgs<T> a = base;
-            fill_geometry<T>(p, p->ax[0], a);
-            a.green = 1;
+            fill_geometry<T>(p, p->ax[g], a);
             a.ncomp = C;
             a.bstride = (int64_t)C * p->V;
             a.src = W;
             a.dst = W;
-            rc = launch_pass<T>(p, p->ax[0], a, nbatch, true, kGreen, stream);
+            rc = launch_pass<T>(p, p->ax[g], a, nbatch, true, kGreen, stream);
             if (rc) return rc;
         }
-        for (int q = 1; q <= d - 2; ++q) {  // middle axes, inverse (1/N folded into G)
+        for (int q = nothers - 1; q >= 0; --q) {  // remaining axes, inverse (1/N folded into G)
             PassArgs<T> a = base;
-            fill_geometry<T>(p, p->ax[q], a);
-            a.dir = 1;
+            fill_geometry<T>(p, p->ax[others[q]], a);
             a.src = W;
             a.dst = W;
-            rc = launch_pass<T>(p, p->ax[q], a, nbatch * C, false, kInv, stream);
+            rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kInv, stream);
             if (rc) return rc;
         }
         {   // last pass: inverse FFT along the innermost axis + Born update + norms
             PassArgs<T> a = base;
             fill_geometry<T>(p, inner, a);
-            a.dir = 1;
-            a.epilogue = 1;
             a.ncomp = C;
             a.bstride = (int64_t)C * p->V;
             a.src = W;
@@ -408,8 +435,8 @@ int grid_for(int64_t n) {
 template <typename T>
 int chi_mul_impl(mmx_plan_s* p, const void* chi, int chi_kind, const void* x, void* y, int C, int64_t nbatch,
                  hipStream_t stream) {
-    if (chi_kind == 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
-    if (chi_kind != 1 && chi_kind != 2) return fail(MMX_EINVAL, "chi_kind must be 1 or 2");
+    if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
+    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
     hipLaunchKernelGGL((chi_mul_kernel<T>), dim3(grid_for(p->V * nbatch)), dim3(256), 0, stream,
                        reinterpret_cast<const cx<T>*>(chi), chi_kind, reinterpret_cast<const cx<T>*>(x),
                        reinterpret_cast<cx<T>*>(y), C, p->V, nbatch);
@@ -457,6 +484,9 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
             v = 65536;
         p->lds_max = (size_t)v;
     }
+    p->g_axis = 0;
+    p->xcd_remap = 0;
+    p->lds_budget = kLdsBudget;
     // non-trivial axes, outermost first
     p->naxes = 0;
     for (int a = 0; a < 3; ++a) {
@@ -504,6 +534,37 @@ int mmx_plan_destroy(mmx_plan_t p) {
     if (p->kvec) (void)hipFree(p->kvec);
     if (p->scratch) (void)hipFree(p->scratch);
     delete p;
+    return MMX_OK;
+}
+
+int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
+    g_last_error.clear();
+    if (!p) return fail(MMX_EINVAL, "null plan");
+    if (value < 0) return fail(MMX_EINVAL, "parameter values are >= 0");
+    switch (param) {
+        case MMX_PARAM_G_AXIS: p->g_axis = (int)value; break;
+        case MMX_PARAM_P_ROWS: p->p_rows = (int)value; break;
+        case MMX_PARAM_P_STRIDED: p->p_strided = (int)value; break;
+        case MMX_PARAM_P_GREEN: p->p_green = (int)value; break;
+        case MMX_PARAM_XCD_REMAP: p->xcd_remap = value ? 1 : 0; break;
+        case MMX_PARAM_LDS_BUDGET: p->lds_budget = value ? (size_t)value : kLdsBudget; break;
+        default: return fail(MMX_EINVAL, "unknown parameter");
+    }
+    return MMX_OK;
+}
+
+int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
+    g_last_error.clear();
+    if (!p || !value) return fail(MMX_EINVAL, "null argument");
+    switch (param) {
+        case MMX_PARAM_G_AXIS: *value = p->g_axis; break;
+        case MMX_PARAM_P_ROWS: *value = p->p_rows; break;
+        case MMX_PARAM_P_STRIDED: *value = p->p_strided; break;
+        case MMX_PARAM_P_GREEN: *value = p->p_green; break;
+        case MMX_PARAM_XCD_REMAP: *value = p->xcd_remap; break;
+        case MMX_PARAM_LDS_BUDGET: *value = (int64_t)p->lds_budget; break;
+        default: return fail(MMX_EINVAL, "unknown parameter");
+    }
     return MMX_OK;
 }
 

@@ -40,6 +40,39 @@ template <typename T> __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
 }
 
 // ----------------------------------------------------------------------------------------------
+// Division by a launch-constant divisor without the integer-divide sequence: a float reciprocal gives the
+// quotient to within one, the remainder test corrects it. Exact for 0 <= x < 2^22.
+// ----------------------------------------------------------------------------------------------
+struct FDiv {
+    int d;
+    float inv;
+};
+
+__host__ inline FDiv make_fdiv(int d) { return FDiv{d, 1.0f / float(d)}; }
+
+__device__ __forceinline__ int fdiv(int x, const FDiv& f) {
+    int q = __float2int_rz(__int2float_rn(x) * f.inv);
+    const int r = x - q * f.d;
+    q += (r >= f.d) - (r < 0);
+    return q;
+}
+
+// 64-bit variant through a double reciprocal (exact for 0 <= x < 2^50).
+struct FDiv64 {
+    int64_t d;
+    double inv;
+};
+
+__host__ inline FDiv64 make_fdiv64(int64_t d) { return FDiv64{d, 1.0 / double(d)}; }
+
+__device__ __forceinline__ int64_t fdiv64(int64_t x, const FDiv64& f) {
+    int64_t q = (int64_t)(double(x) * f.inv);
+    const int64_t r = x - q * f.d;
+    q += (r >= f.d) - (r < 0);
+    return q;
+}
+
+// ----------------------------------------------------------------------------------------------
 // Butterflies: y_j = sum_m a_m exp(DIR 2 pi i j m / R), in place.
 // ----------------------------------------------------------------------------------------------
 template <typename T, int DIR> __device__ __forceinline__ void dft2(cx<T>* a) {
@@ -151,15 +184,25 @@ struct TileGeom {
     int M;       // pencils in the tile
     int SJ, SM;  // LDS strides
     int m_fast;  // 1: consecutive threads take consecutive pencils
+    FDiv fM;     // division by M
+};
+
+// Per-radix-pass constants of one FFT length (host-computed): butterflies per pencil and stride product.
+struct RadixPlan {
+    int nrad;
+    int rad[kMaxRadices];
+    FDiv per[kMaxRadices];  // N / R
+    FDiv p[kMaxRadices];    // product of the radices before this pass
 };
 
 // One Stockham pass of radix R over all pencils of the tile. Each thread owns at most
 // Q = ceil(E/R) butterflies (the host sizes the block so that M*N <= E*blockDim).
 template <typename T, int R, int DIR, int E>
-__device__ __forceinline__ void stockham_pass(cx<T>* lds, const cx<T>* tw, const TileGeom& g, int p,
-                                              int tid, int nth) {
+__device__ __forceinline__ void stockham_pass(cx<T>* lds, const cx<T>* tw, const TileGeom& g, const FDiv& fper,
+                                              const FDiv& fp, int tid, int nth) {
     constexpr int Q = (E + R - 1) / R;
-    const int per = g.N / R;        // butterflies per pencil
+    const int per = fper.d;         // butterflies per pencil
+    const int p = fp.d;
     const int total = g.M * per;
     const int twstride = g.N / (p * R);
     cx<T> v[Q][R];
@@ -170,9 +213,9 @@ __device__ __forceinline__ void stockham_pass(cx<T>* lds, const cx<T>* tw, const
         obase[q] = -1;
         if (b < total) {
             int m, i;
-            if (g.m_fast) { m = b % g.M; i = b / g.M; }
-            else { i = b % per; m = b / per; }
-            const int k = i % p;
+            if (g.m_fast) { i = fdiv(b, g.fM); m = b - i * g.M; }
+            else { m = fdiv(b, fper); i = b - m * per; }
+            const int k = i - fdiv(i, fp) * p;
             const int base = m * g.SM;
 #pragma unroll
             for (int j = 0; j < R; ++j) v[q][j] = lds[(i + j * per) * g.SJ + base];
@@ -201,21 +244,17 @@ __device__ __forceinline__ void stockham_pass(cx<T>* lds, const cx<T>* tw, const
 
 // Full FFT of every pencil of the tile (natural order in, natural order out, unnormalised).
 template <typename T, int DIR, int E>
-__device__ void tile_fft(cx<T>* lds, const cx<T>* tw, const TileGeom& g, const int* rad, int nrad,
-                         int tid, int nth) {
-    int p = 1;
-    for (int r = 0; r < nrad; ++r) {
-        const int R = rad[r];
-        switch (R) {
-            case 8: stockham_pass<T, 8, DIR, E>(lds, tw, g, p, tid, nth); break;
-            case 4: stockham_pass<T, 4, DIR, E>(lds, tw, g, p, tid, nth); break;
-            case 2: stockham_pass<T, 2, DIR, E>(lds, tw, g, p, tid, nth); break;
-            case 3: stockham_pass<T, 3, DIR, E>(lds, tw, g, p, tid, nth); break;
-            case 5: stockham_pass<T, 5, DIR, E>(lds, tw, g, p, tid, nth); break;
-            case 7: stockham_pass<T, 7, DIR, E>(lds, tw, g, p, tid, nth); break;
+__device__ void tile_fft(cx<T>* lds, const cx<T>* tw, const TileGeom& g, const RadixPlan& rp, int tid, int nth) {
+    for (int r = 0; r < rp.nrad; ++r) {
+        switch (rp.rad[r]) {
+            case 8: stockham_pass<T, 8, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
+            case 4: stockham_pass<T, 4, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
+            case 2: stockham_pass<T, 2, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
+            case 3: stockham_pass<T, 3, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
+            case 5: stockham_pass<T, 5, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
+            case 7: stockham_pass<T, 7, DIR, E>(lds, tw, g, rp.per[r], rp.p[r], tid, nth); break;
             default: break;  // rejected by the host plan
         }
-        p *= R;
     }
 }
 

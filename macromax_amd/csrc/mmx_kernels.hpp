@@ -32,13 +32,16 @@ template <typename T>
 struct PassArgs {
     // axis geometry (per component, per problem)
     int N;
-    int nrad;
-    int rad[kMaxRadices];
+    RadixPlan rp;             // radices and their division constants
+    FDiv fN, fP, fncomp;      // division by N, P, ncomp
+    FDiv64 fn1, fn2, fntile;  // unravel of voxel indices; tile index -> (outer, inner tile)
+    FDiv64 ftpp;              // division by tiles_per_problem
     int64_t inner;            // element stride of the axis (1: rows)
     int64_t nouter;           // outer blocks per component per problem (rows: number of rows)
     int P;                    // pencils per tile and component
     int ntile_inner;          // strided tiles along the inner extent
     int64_t tiles_per_problem;
+    int xcd_remap;            // 1: consecutive tiles run on the same XCD (blocks are dealt to 8 XCDs round-robin)
     int ncomp;                // components held per tile (1 when components are folded into nouter)
     int64_t comp_stride;      // V, elements between components
     int64_t bstride;          // elements between problems (C*V) for src/dst in fused passes
@@ -53,7 +56,7 @@ struct PassArgs {
     int epilogue;             // 0: store dst; 1: Born update into Eout + norms
     // Born operands
     int C;                    // field components (1 or 3)
-    int chi_kind;             // 1: [1,1,*g]; 2: [3,3,*g]
+    int chi_kind;             // 1: [1,1,*g]; 2: [3,3,*g]; 3: symmetric [6,*g]
     const cx<T>* E;
     const cx<T>* chi;
     const cx<T>* s;
@@ -108,13 +111,14 @@ __device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, b
 
 template <typename T>
 __device__ __forceinline__ void unravel3(int64_t v, const PassArgs<T>& a, int64_t& i0, int64_t& i1, int64_t& i2) {
-    i2 = v % a.n2;
-    const int64_t t = v / a.n2;
-    i1 = t % a.n1;
-    i0 = t / a.n1;
+    const int64_t t = fdiv64(v, a.fn2);
+    i2 = v - t * a.n2;
+    i0 = fdiv64(t, a.fn1);
+    i1 = t - i0 * a.n1;
 }
 
-// chi_eff applied to the C-vector x at voxel v of problem b.
+// chi_eff applied to the C-vector x at voxel v of problem b. chi_kind 1: one value per voxel; 2: 3x3 row-major
+// [9, V]; 3: complex-symmetric 3x3 packed as [xx, xy, xz, yy, yz, zz] ([6, V]).
 template <typename T>
 __device__ __forceinline__ void chi_apply(const PassArgs<T>& a, const cx<T>* chi_b, int64_t v, const cx<T>* x,
                                           cx<T>* y, const Coef<T>& cf) {
@@ -123,7 +127,7 @@ __device__ __forceinline__ void chi_apply(const PassArgs<T>& a, const cx<T>* chi
         const cx<T> x0 = chi_b[v];
 #pragma unroll
         for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = scal(x0 * x[c], cf.a) + cc * x[c];
-    } else {
+    } else if (a.chi_kind == 2) {
         const int64_t V = a.comp_stride;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -132,6 +136,16 @@ __device__ __forceinline__ void chi_apply(const PassArgs<T>& a, const cx<T>* chi
             acc = acc + chi_b[(r * 3 + 2) * V + v] * x[2];
             y[r] = scal(acc, cf.a) + cc * x[r];
         }
+    } else {
+        const int64_t V = a.comp_stride;
+        const cx<T> xx = chi_b[v], xy = chi_b[V + v], xz = chi_b[2 * V + v];
+        const cx<T> yy = chi_b[3 * V + v], yz = chi_b[4 * V + v], zz = chi_b[5 * V + v];
+        const cx<T> y0 = xx * x[0] + xy * x[1] + xz * x[2];
+        const cx<T> y1 = xy * x[0] + yy * x[1] + yz * x[2];
+        const cx<T> y2 = xz * x[0] + yz * x[1] + zz * x[2];
+        y[0] = scal(y0, cf.a) + cc * x[0];
+        y[1] = scal(y1, cf.a) + cc * x[1];
+        y[2] = scal(y2, cf.a) + cc * x[2];
     }
 }
 
@@ -160,9 +174,14 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     const bool rows = (a.inner == 1);
 
     // which problem and which tile
-    const int64_t blk = blockIdx.x;
-    const int64_t b = blk / a.tiles_per_problem;
-    const int64_t t = blk % a.tiles_per_problem;
+    int64_t blk = blockIdx.x;
+    if (a.xcd_remap) {
+        // block i runs on XCD i % 8; give each XCD a contiguous range of tiles so neighbouring tiles share an L2
+        const int64_t n = gridDim.x, q = n >> 3, rem = n & 7, r = blk & 7;
+        blk = r * q + (r < rem ? r : rem) + (blk >> 3);
+    }
+    const int64_t b = fdiv64(blk, a.ftpp);
+    const int64_t t = blk - b * a.tiles_per_problem;
     const Coef<T> cf = (a.coef != nullptr) ? a.coef[b] : Coef<T>{T(0), T(1), T(1), T(0), T(0), T(0)};
 
     TileGeom g;
@@ -170,13 +189,15 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     g.M = a.ncomp * a.P;
     if (rows) { g.SJ = 1; g.SM = N + a.pad; g.m_fast = 0; }
     else { g.SJ = g.M; g.SM = 1; g.m_fast = 1; }
+    g.fM.d = g.M;
+    g.fM.inv = 1.0f / float(g.M);
 
     for (int q = tid; q < N; q += nth) tw[q] = a.tw[q];
 
     // tile origin: rows -> first row r0; strided -> (outer block o, first inner index i0)
     int64_t r0 = 0, o = 0, ib = 0;
     if (rows) r0 = t * a.P;
-    else { o = t / a.ntile_inner; ib = (t % a.ntile_inner) * a.P; }
+    else { o = fdiv64(t, a.fntile); ib = (t - o * a.ntile_inner) * a.P; }
     const int64_t pb = b * a.bstride;  // problem offset in src/dst (fused passes; 0 when folded)
 
     // ---------------- load ----------------
@@ -188,8 +209,8 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         const int64_t V = a.comp_stride;
         const int nvox = a.P * N;
         for (int idx = tid; idx < nvox; idx += nth) {
-            const int j = idx % N;
-            const int p = idx / N;
+            const int p = fdiv(idx, a.fN);
+            const int j = idx - p * N;
             const int64_t row = r0 + p;
             cx<T> y[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
             if (row < a.nouter) {
@@ -208,10 +229,10 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
       if (rows) {
         const int total = a.ncomp * a.P * N;
         for (int idx = tid; idx < total; idx += nth) {
-            const int j = idx % N;
-            const int rest = idx / N;
-            const int p = rest % a.P;
-            const int c = rest / a.P;
+            const int rest = fdiv(idx, a.fN);
+            const int j = idx - rest * N;
+            const int c = fdiv(rest, a.fP);
+            const int p = rest - c * a.P;
             const int64_t row = r0 + p;
             cx<T> val{T(0), T(0)};
             if (row < a.nouter) val = a.src[pb + c * a.comp_stride + row * N + j];
@@ -221,10 +242,10 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         const int total = a.ncomp * a.P * N;
         const int64_t obase = pb + o * (int64_t)N * a.inner + ib;
         for (int idx = tid; idx < total; idx += nth) {
-            const int p = idx % a.P;
-            const int rest = idx / a.P;
-            const int c = rest % a.ncomp;
-            const int j = rest / a.ncomp;
+            const int rest = fdiv(idx, a.fP);
+            const int p = idx - rest * a.P;
+            const int j = fdiv(rest, a.fncomp);
+            const int c = rest - j * a.ncomp;
             cx<T> val{T(0), T(0)};
             if (ib + p < a.inner) val = a.src[obase + c * a.comp_stride + j * a.inner + p];
             tile[idx] = val;  // == j*M + c*P + p
@@ -235,13 +256,13 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
 
     // ---------------- FFT (and Green) ----------------
     if constexpr (GREEN) {
-        tile_fft<T, -1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+        tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
         const int nvox = a.P * N;
         for (int idx = tid; idx < nvox; idx += nth) {
             int p, j;
             int64_t v;
-            if (rows) { j = idx % N; p = idx / N; v = (r0 + p) * N + j; if (r0 + p >= a.nouter) continue; }
-            else { p = idx % a.P; j = idx / a.P; if (ib + p >= a.inner) continue; v = (o * N + j) * a.inner + ib + p; }
+            if (rows) { p = fdiv(idx, a.fN); j = idx - p * N; v = (r0 + p) * N + j; if (r0 + p >= a.nouter) continue; }
+            else { j = fdiv(idx, a.fP); p = idx - j * a.P; if (ib + p >= a.inner) continue; v = (o * N + j) * a.inner + ib + p; }
             int64_t i0, i1, i2;
             unravel3(v, a, i0, i1, i2);
             const T k0 = a.kvec[i0];
@@ -255,11 +276,11 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
             for (int c = 0; c < 3; ++c) if (c < a.ncomp) tile[j * g.SJ + (c * a.P + p) * g.SM] = f[c];
         }
         __syncthreads();
-        tile_fft<T, +1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+        tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
     } else if constexpr (KIND == kFwd || KIND == kProFwd) {
-        tile_fft<T, -1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+        tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
     } else {
-        tile_fft<T, +1, E>(tile, tw, g, a.rad, a.nrad, tid, nth);
+        tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
     }
 
     // ---------------- store ----------------
@@ -272,8 +293,8 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         double dd = 0.0, ee = 0.0;
         const int nvox = a.P * N;
         for (int idx = tid; idx < nvox; idx += nth) {
-            const int j = idx % N;
-            const int p = idx / N;
+            const int p = fdiv(idx, a.fN);
+            const int j = idx - p * N;
             const int64_t row = r0 + p;
             if (row >= a.nouter) continue;
             const int64_t v = row * N + j;
@@ -303,10 +324,10 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
       if (rows) {
         const int total = a.ncomp * a.P * N;
         for (int idx = tid; idx < total; idx += nth) {
-            const int j = idx % N;
-            const int rest = idx / N;
-            const int p = rest % a.P;
-            const int c = rest / a.P;
+            const int rest = fdiv(idx, a.fN);
+            const int j = idx - rest * N;
+            const int c = fdiv(rest, a.fP);
+            const int p = rest - c * a.P;
             const int64_t row = r0 + p;
             if (row < a.nouter) a.dst[pb + c * a.comp_stride + row * N + j] = scal(tile[(c * a.P + p) * g.SM + j], a.scale);
         }
@@ -314,10 +335,10 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         const int total = a.ncomp * a.P * N;
         const int64_t obase = pb + o * (int64_t)N * a.inner + ib;
         for (int idx = tid; idx < total; idx += nth) {
-            const int p = idx % a.P;
-            const int rest = idx / a.P;
-            const int c = rest % a.ncomp;
-            const int j = rest / a.ncomp;
+            const int rest = fdiv(idx, a.fP);
+            const int p = idx - rest * a.P;
+            const int j = fdiv(rest, a.fncomp);
+            const int c = rest - j * a.ncomp;
             if (ib + p < a.inner) a.dst[obase + c * a.comp_stride + j * a.inner + p] = scal(tile[idx], a.scale);
         }
       }
@@ -361,27 +382,32 @@ __global__ void __launch_bounds__(256) sum_kernel(const double* partials, int64_
     if (threadIdx.x == 0) out[0] = s;
 }
 
-// y = chi x per voxel (chi [1,1,V] or [3,3,V]; x, y [C,V] with batch strides), no affine coefficients.
+// y = chi x per voxel (chi [1,1,V], [3,3,V] or packed symmetric [6,V]; x, y [C,V] with batch strides).
 template <typename T>
 __global__ void __launch_bounds__(256) chi_mul_kernel(const cx<T>* chi, int chi_kind, const cx<T>* x, cx<T>* y,
                                                       int C, int64_t V, int64_t nbatch) {
     const int64_t total = V * nbatch;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
         const int64_t b = q / V;
-        const int64_t v = q % V;
+        const int64_t v = q - b * V;
         const cx<T>* xb = x + b * C * V;
         cx<T>* yb = y + b * C * V;
         if (chi_kind == 1) {
             const cx<T> c0 = chi[v];
             for (int c = 0; c < C; ++c) yb[c * V + v] = c0 * xb[c * V + v];
         } else {
-            cx<T> xv[3] = {xb[v], xb[V + v], xb[2 * V + v]};
-            cx<T> yv[3];
+            const cx<T> xv[3] = {xb[v], xb[V + v], xb[2 * V + v]};
+            cx<T> m[9];
+            if (chi_kind == 2) {
 #pragma unroll
-            for (int r = 0; r < 3; ++r)
-                yv[r] = chi[(r * 3) * V + v] * xv[0] + chi[(r * 3 + 1) * V + v] * xv[1] + chi[(r * 3 + 2) * V + v] * xv[2];
+                for (int e = 0; e < 9; ++e) m[e] = chi[e * V + v];
+            } else {
+                const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
 #pragma unroll
-            for (int r = 0; r < 3; ++r) yb[r * V + v] = yv[r];
+                for (int e = 0; e < 9; ++e) m[e] = chi[map[e] * V + v];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) yb[r * V + v] = m[3 * r] * xv[0] + m[3 * r + 1] * xv[1] + m[3 * r + 2] * xv[2];
         }
     }
 }

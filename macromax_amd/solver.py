@@ -197,8 +197,18 @@ class Solution:
         else:
             chi = chi - torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(3, 3, *([1] * self.grid.ndim)) \
                 * (alpha * 1.0j / alpha.imag)
-        self.__chi = chi.expand(chi.shape[0], chi.shape[1], *full).to(BE.hardware_dtype).contiguous()
-        self.__chi_kind = 1 if chi.shape[0] == 1 else 2
+        chi = chi.expand(chi.shape[0], chi.shape[1], *full)
+        if chi.shape[0] == 1:
+            self.__chi_kind = 1
+            self.__chi = chi.to(BE.hardware_dtype).contiguous()
+        elif torch.equal(eps64, eps64.transpose(0, 1)):
+            # reciprocal medium: chi^T = chi, stored as the 6 distinct entries {xx, xy, xz, yy, yz, zz}
+            self.__chi_kind = 3
+            self.__chi = torch.stack([chi[0, 0], chi[0, 1], chi[0, 2], chi[1, 1], chi[1, 2], chi[2, 2]]).to(
+                BE.hardware_dtype).contiguous()
+        else:
+            self.__chi_kind = 2
+            self.__chi = chi.to(BE.hardware_dtype).contiguous()
         self.__coef = torch.empty(6, dtype=BE.real_dtype, device=BE.device)
         self.__upload_coef()
         self.source_distribution = source_distribution
@@ -279,13 +289,23 @@ class Solution:
         return self.__beta
 
     @property
+    def chi_kind(self) -> int:
+        """Storage of chi on the device: 1 isotropic, 2 full 3x3, 3 complex-symmetric 3x3 (6 entries)."""
+        return self.__chi_kind
+
+    @property
     def chi(self) -> torch.Tensor:
-        """The stored susceptibility chi (for the alpha chosen at set-up; see ``chi_effective``)."""
-        return self.__chi
+        """The susceptibility chi for the alpha chosen at set-up, as a [1,1,*grid] or [3,3,*grid] tensor."""
+        if self.__chi_kind != 3:
+            return self.__chi
+        c = self.__chi
+        return torch.stack([torch.stack([c[0], c[1], c[2]]), torch.stack([c[1], c[3], c[4]]),
+                            torch.stack([c[2], c[4], c[5]])])
 
     @property
     def chi_effective(self) -> torch.Tensor:
-        c = self.__chi * self.__coef_a
+        """chi for the current alpha: a chi + c I (the operator the kernels apply)."""
+        c = self.chi * self.__coef_a
         if self.__coef_c != 0:
             if c.shape[0] == 1:
                 c = c + self.__coef_c

@@ -1,0 +1,206 @@
+"""BackEndHip conformance: the reference's backend test-suite (tests/test_backend.py:11-343) on the HIP backend.
+
+Same grid (5x10x20, pitch 100 nm, k0 = 1e7, so the k0-scaled pitch is 1), same known answers. The 5x10x20 grid
+exercises radix-5 and mixed-radix Stockham passes.
+"""
+import numpy as np
+import numpy.testing as npt
+import pytest
+import scipy.linalg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def BE(device):
+    from macromax_amd.backend.hip import BackEndHip
+    from macromax_amd.grid import Grid
+    grid = Grid([5, 10, 20], 100e-9)
+    return BackEndHip(3, grid * 10e6, hardware_dtype=np.complex64)
+
+
+def test_ft_axes(BE):
+    npt.assert_equal(BE.ft_axes, [2, 3, 4])
+
+
+def test_assign_exact_and_assign(BE):
+    x = BE.allocate_array()
+    x_np = BE.astype(np.arange(np.prod(x.shape)).reshape(x.shape))
+    x = BE.assign_exact(x_np, x)
+    npt.assert_array_equal(BE.asnumpy(x), BE.asnumpy(x_np))
+    x_np = np.arange(np.prod(x.shape)).reshape(x.shape).astype(BE.numpy_dtype)
+    x = BE.assign(x_np, x)
+    npt.assert_array_equal(BE.asnumpy(x), x_np)
+    x = BE.assign(0, x)
+    npt.assert_array_equal(BE.asnumpy(x), 0)
+    x = BE.assign(1, x)
+    npt.assert_array_equal(BE.asnumpy(x), 1)
+
+
+def test_allclose_amax_sort(BE):
+    arr = BE.assign(0.0, BE.allocate_array())
+    other = BE.assign(0, BE.allocate_array())
+    assert BE.allclose(arr, other)
+    other = BE.assign(1e-9, other)
+    assert BE.allclose(arr, other)
+    other = BE.assign(0.1, other)
+    assert not BE.allclose(arr, other)
+    sh = [3, *BE.grid.shape]
+    npt.assert_array_equal(BE.amax(BE.astype(np.arange(np.prod(sh)).reshape(sh))), np.prod(sh) - 1)
+    t = [[[4, 10, 26], [11, 1, 6], [20, 8, 0]], [[9, 2, 12], [13, 18, 7], [15, 16, 3]],
+         [[5, 23, 19], [24, 17, 22], [21, 25, 14]]]
+    npt.assert_array_equal(BE.asnumpy(BE.sort(BE.astype(t))).real, np.sort(t, axis=0))
+
+
+def test_dtype_astype_eps(BE):
+    import torch
+    assert BE.hardware_dtype == torch.complex64
+    npt.assert_equal(BE.asnumpy(BE.astype([1, 2, 3])), np.array([1, 2, 3], dtype=np.complex64))
+    npt.assert_equal(BE.asnumpy(BE.astype([[1, 2, 3]])), np.array([[1, 2, 3]], dtype=np.complex64))
+    npt.assert_equal(BE.eps, np.finfo(np.complex64).eps)
+
+
+def test_allocate_eye_vector_length_grid(BE):
+    from macromax_amd.grid import Grid
+    npt.assert_array_equal(BE.allocate_array().shape, BE.array_ft_input.shape)
+    npt.assert_array_equal(BE.eye.shape, [3, 3, 1, 1, 1])
+    npt.assert_array_equal(BE.asnumpy(BE.eye[:, :, 0, 0, 0]), np.eye(3))
+    assert BE.vector_length == 3 and BE.vectorial
+    assert BE.grid == Grid([5, 10, 20], 100e-9) * 10e6
+    npt.assert_array_almost_equal(BE.grid.step, (1, 1, 1))
+
+
+def test_isscalar_and_to_matrix_field(BE):
+    assert BE.is_scalar(BE.astype(np.pi))
+    assert BE.is_scalar(BE.astype(np.exp(1j)))
+    assert BE.is_scalar(BE.astype(np.array([1.0])))
+    assert BE.is_scalar(BE.astype(np.array([[1.0]])))
+    assert BE.is_scalar(BE.astype(np.array([[[1.0]]])))
+    assert not BE.is_scalar(BE.astype(np.array([1.0, 1.0])))
+    npt.assert_equal(BE.asnumpy(BE.to_matrix_field(2)), 2)
+    npt.assert_array_equal(BE.to_matrix_field(np.arange(3)).shape, (1, 1, 1, 1, 3))
+    A = BE.to_matrix_field(np.tile(np.arange(3)[:, None, None, None], [1, 5, 10, 20]))
+    assert A.ndim == 5 and tuple(A.shape[:2]) == (3, 1)
+
+
+def test_ft_and_ift_known_answers(BE):
+    """tests/test_backend.py:162-180: delta -> ones (decimal 14 in the reference; complex64 here), 1/N inverse."""
+    A = np.zeros([3, 3, 5, 10, 20])
+    A[:, :, 0, 0, 0] = 1
+    npt.assert_almost_equal(BE.asnumpy(BE.ft(BE.astype(A))), np.ones([3, 3, 5, 10, 20]), decimal=6)
+    B_np = np.arange(9 * 1000).reshape((3, 3, 5, 10, 20))
+    back = BE.asnumpy(BE.ift(BE.ft(BE.astype(B_np))))
+    npt.assert_array_almost_equal(back, B_np, decimal=2)
+    assert np.linalg.norm(back - B_np) / np.linalg.norm(B_np) < 1e-6
+    npt.assert_array_almost_equal(BE.asnumpy(BE.ift(BE.astype(A))), np.ones([3, 3, 5, 10, 20]) / 1000)
+    F = np.fft.fftn(B_np, axes=(2, 3, 4))
+    got = BE.asnumpy(BE.ft(BE.astype(B_np)))
+    assert np.linalg.norm(got - F) / np.linalg.norm(F) < 2e-6
+
+
+def test_conjugate_transpose_subtract_mul(BE):
+    A = BE.astype(np.array([[[[[1.0]]]], [[[[1.0j]]]], [[[[0.0]]]]]))
+    npt.assert_equal(BE.asnumpy(BE.adjoint(A)), np.array([[[[[1.0]]], [[[-1.0j]]], [[[0.0]]]]]))
+    npt.assert_equal(BE.asnumpy(BE.subtract(3 * BE.eye, 2)), BE.asnumpy(BE.eye))
+    A = BE.astype(np.arange(9).reshape((3, 3, 1, 1, 1)))
+    B = BE.astype(np.arange(8, -1, -1).reshape((3, 3, 1, 1, 1)))
+    npt.assert_array_equal(BE.asnumpy(BE.subtract(A, B)), 2 * np.arange(9).reshape((3, 3, 1, 1, 1)) - 8)
+    V = BE.astype(np.arange(3).reshape((3, 1, 1, 1, 1)))
+    npt.assert_equal(BE.asnumpy(BE.mul(A, V)), BE.asnumpy(A[:, 1:2] + 2 * A[:, 2:3]))
+    AF = BE.astype(np.tile(BE.asnumpy(A), [1, 1, 5, 10, 20]))
+    npt.assert_equal(BE.asnumpy(BE.mul(AF, V)), BE.asnumpy(A[:, 1:2] + 2 * AF[:, 2:3]))
+    npt.assert_equal(BE.asnumpy(BE.mul(A, 2)), BE.asnumpy(2 * A))
+    npt.assert_equal(BE.asnumpy(BE.mul(3, A)), BE.asnumpy(3 * A))
+    npt.assert_equal(BE.asnumpy(BE.mul(A, B)),
+                     (BE.asnumpy(A)[:, :, 0, 0, 0] @ BE.asnumpy(B)[:, :, 0, 0, 0])[:, :, None, None, None])
+    # full-grid 3x3 field times vector field: the HIP chi_mul kernel
+    rng = np.random.default_rng(0)
+    chi = rng.standard_normal((3, 3, 5, 10, 20)) + 1j * rng.standard_normal((3, 3, 5, 10, 20))
+    x = rng.standard_normal((3, 1, 5, 10, 20)) + 1j * rng.standard_normal((3, 1, 5, 10, 20))
+    got = BE.asnumpy(BE.mul(BE.astype(chi), BE.astype(x)))
+    npt.assert_allclose(got, np.einsum('ij...,jk...->ik...', chi, x), rtol=1e-5, atol=1e-5)
+
+
+def test_ldivide_inv(BE):
+    A = BE.astype(np.arange(9).reshape((3, 3, 1, 1, 1)) ** 2)
+    npt.assert_equal(BE.asnumpy(BE.ldivide(2, A)), BE.asnumpy(0.5 * A))
+    B = np.array([[1.0, 2j, 3], [-4j, 5, 6], [7, 8, 9]])[:, :, None, None, None]
+    X = BE.ldivide(A, BE.astype(B))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.mul(A, X)), B, decimal=4)
+    Bt = BE.astype(B)
+    npt.assert_array_almost_equal(BE.asnumpy(BE.mul(BE.inv(Bt), Bt)), BE.asnumpy(BE.eye), decimal=5)
+
+
+def test_curl_ft(BE):
+    A = np.zeros([3, 1, 5, 10, 20])
+    A[:, 0, 0, 0, 5] = [3.0, 0, 0]
+    C = np.zeros([3, 1, 5, 10, 20], dtype=np.complex128)
+    C[:, 0, 0, 0, 5] = [0, 2 * np.pi * 0.25j * 3.0, 0]
+    npt.assert_almost_equal(BE.asnumpy(BE.curl_ft(BE.astype(A))), C, decimal=5)
+
+
+@pytest.mark.parametrize('idx,val,expected', [((0, 0, 10), [2, 3, 4], [0, 0, 4]), ((0, 0, 2), [2, 3, 4], [0, 0, 4]),
+                                              ((0, 5, 10), [4, 5, 6], [0, 5.5, 5.5])])
+def test_longitudinal_projection_ft(BE, idx, val, expected):
+    A = np.zeros([3, 1, 5, 10, 20])
+    A[(slice(None), 0) + idx] = val
+    C = np.zeros_like(A)
+    C[(slice(None), 0) + idx] = expected
+    npt.assert_array_almost_equal(BE.asnumpy(BE.longitudinal_projection_ft(BE.astype(A))), C, decimal=5)
+
+
+@pytest.mark.parametrize('idx,val,expected', [((0, 0, 10), [2, 3, 4], [2, 3, 0]), ((0, 0, 3), [2, 3, 4], [2, 3, 0]),
+                                              ((0, 5, 10), [0, 3, 3], [0, 0, 0])])
+def test_transversal_projection_ft(BE, idx, val, expected):
+    A = np.zeros([3, 1, 5, 10, 20])
+    A[(slice(None), 0) + idx] = val
+    C = np.zeros_like(A)
+    C[(slice(None), 0) + idx] = expected
+    npt.assert_array_almost_equal(BE.asnumpy(BE.transversal_projection_ft(BE.astype(A))), C, decimal=5)
+
+
+def test_k_and_k2(BE):
+    from macromax_amd.grid import Grid
+    grid = Grid([5, 10, 20], 100e-9)
+    k = grid.k / 10e6
+    for a in range(3):
+        npt.assert_almost_equal(BE.asnumpy(BE.k[a]), k[a], decimal=5)
+    A = BE.asnumpy(BE.k2)
+    npt.assert_almost_equal(A[1, 1, 1] / (2 * np.pi) ** 2, np.sum(1.0 / np.array([5, 10, 20]) ** 2), decimal=6)
+
+
+def test_mat3_eigh(BE):
+    def srt(a):
+        sh = a.shape
+        return np.asarray(sorted(sorted(a.ravel(), key=np.imag), key=np.real)).reshape(sh)
+    D = BE.astype(np.diag([4, 5, -6])[:, :, None, None, None])
+    npt.assert_almost_equal(srt(BE.asnumpy(BE.mat3_eigh(D))), np.array([-6, 4, 5]).reshape(3, 1, 1, 1), decimal=5)
+    Q = scipy.linalg.orth(np.arange(9).reshape([3, 3]) ** 2 + 1j * np.arange(9).reshape([3, 3]))
+    QD = BE.astype((Q @ np.diag([4, 5, -6]) @ np.conj(Q.T))[:, :, None, None, None])
+    npt.assert_array_almost_equal(srt(BE.asnumpy(BE.mat3_eigh(QD))), np.array([-6, 4, 5]).reshape(3, 1, 1, 1), decimal=5)
+    nI = BE.astype(2.7 * np.eye(3)[:, :, None, None, None])
+    npt.assert_array_almost_equal(BE.asnumpy(BE.mat3_eigh(nI)), 2.7 * np.ones((3, 1, 1, 1)), decimal=5)
+
+    def rot_x_d(a):
+        return 1j * np.array([[0, 0, 0], [0, 0, -a], [0, a, 0]])[:, :, None, None, None]
+    for a in (0.1, 0.01):
+        npt.assert_array_almost_equal(np.sort(BE.asnumpy(BE.mat3_eigh(BE.astype(rot_x_d(a))))[:, 0, 0, 0].real),
+                                      np.sort(np.linalg.eig(rot_x_d(a)[:, :, 0, 0, 0])[0].real), decimal=6)
+
+
+def test_norm_fp64(BE):
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((3, 1, 5, 10, 20)) + 1j * rng.standard_normal((3, 1, 5, 10, 20))).astype(np.complex64)
+    npt.assert_allclose(BE.norm(BE.astype(x)), np.linalg.norm(x.astype(np.complex128).ravel()), rtol=1e-12)
+
+
+def test_load_selects_hip(device):
+    from macromax_amd import backend
+    from macromax_amd.backend.hip import BackEndHip
+    from macromax_amd.grid import Grid
+    be = backend.load(3, Grid([8, 8, 8], 1.0), np.complex64)
+    assert isinstance(be, BackEndHip)
+    be = backend.load(1, Grid([8, 8], 1.0), np.complex128, config_list=[{'type': 'numpy'}, {'type': 'hip'}])
+    assert isinstance(be, BackEndHip) and not be.vectorial
+    with pytest.raises(ValueError):
+        backend.load(1, Grid([11, 8], 1.0), np.complex64)  # prime factor 11: explicit error, no fallback

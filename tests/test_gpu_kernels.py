@@ -63,10 +63,18 @@ def _born_case(shape, C, kind, nbatch=1, seed=0, chi_shared=True):
         chi = 0.3 * crand(nch, 1, 1, *shape)
     else:
         chi = 0.2 * crand(nch, 3, 3, *shape)
+        if kind == 3:
+            chi = 0.5 * (chi + np.swapaxes(chi, 1, 2))  # complex-symmetric
     alphas = [complex(1.1 + 0.2 * b, 0.35 + 0.05 * b) for b in range(nbatch)]
     a = [1.0 - 0.1 * b for b in range(nbatch)]
     c = [complex(0.05 * b, -0.02 * b) for b in range(nbatch)]
     return E, s, chi, alphas, a, c
+
+
+def _pack(chi, kind):
+    if kind != 3:
+        return chi
+    return np.stack([chi[0, 0], chi[0, 1], chi[0, 2], chi[1, 1], chi[1, 2], chi[2, 2]])
 
 
 def _oracle_update(E, s, chi, alpha, a, c, k, vectorial):
@@ -82,6 +90,7 @@ BORN_CASES = [
     ((256, 256), 1, 1), ((64, 96), 3, 1), ((48, 40), 3, 2), ((1, 64), 3, 2),
     ((32, 32, 32), 1, 1), ((24, 32, 40), 3, 1), ((16, 20, 24), 3, 2), ((5, 10, 20), 3, 2),
     ((14, 21, 8), 3, 1), ((8, 1, 12), 3, 2), ((64, 64, 64), 3, 2),
+    ((16, 20, 24), 3, 3), ((64, 48), 3, 3), ((512,), 3, 3),
 ]
 
 
@@ -102,7 +111,7 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
     Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
     Eo = torch.zeros_like(Et)
     st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
-    ct = torch.as_tensor(chi[0], dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind), dtype=tdt, device=device).contiguous()
     coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
     norms = torch.zeros(2, dtype=torch.float64, device=device)
     plan.born_step(Et, Eo, st, ct, C, kind, coef, 1, 0, 0, norms)
@@ -119,7 +128,8 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
 
 
 @pytest.mark.parametrize('shape,C,kind,shared', [((64, 48), 1, 1, True), ((12, 16, 20), 3, 2, True),
-                                                 ((12, 16, 20), 3, 1, False), ((512,), 3, 2, False)],
+                                                 ((12, 16, 20), 3, 1, False), ((512,), 3, 2, False),
+                                                 ((12, 16, 20), 3, 3, False)],
                          ids=lambda v: str(v))
 def test_born_step_batched(shape, C, kind, shared, device):
     import torch
@@ -132,12 +142,12 @@ def test_born_step_batched(shape, C, kind, shared, device):
     Et = torch.as_tensor(E, dtype=torch.complex64, device=device).contiguous()
     Eo = torch.zeros_like(Et)
     st = torch.as_tensor(s, dtype=torch.complex64, device=device).contiguous()
-    ct = torch.as_tensor(chi, dtype=torch.complex64, device=device).contiguous()
+    ct = torch.as_tensor(np.stack([_pack(_, kind) for _ in chi]), dtype=torch.complex64, device=device).contiguous()
     coef = torch.tensor([[al.real, al.imag, aa, cc.real, cc.imag, 0.0] for al, aa, cc in zip(alphas, a, c)],
                         dtype=torch.float32, device=device)
     norms = torch.zeros(2 * nb, dtype=torch.float64, device=device)
     V = int(np.prod(shape))
-    chi_size = (1 if kind == 1 else 9) * V
+    chi_size = {1: 1, 2: 9, 3: 6}[kind] * V
     plan.born_step(Et, Eo, st, ct, C, kind, coef, nb, C * V, 0 if shared else chi_size, norms)
     torch.cuda.synchronize()
     out = Eo.cpu().numpy()

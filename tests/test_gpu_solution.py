@@ -1,0 +1,222 @@
+"""The Solution API on the HIP backend: the reference's solver tests (tests/test_solution.py) and per-iteration
+parity of the field trajectory with the complex128 oracle.
+
+Tolerances: analytic plane waves as in the reference (residue < 1e-6 within 70 / 85 iterations, field error
+decimal=3 in the interior and decimal=2 overall); trajectory E_k against the oracle started from the same alpha,
+relative L2 <= 1e-5 for complex64 after every one of the first 10 iterations (<= 1e-10 for complex128).
+"""
+import numpy as np
+import numpy.testing as npt
+import pytest
+import scipy.constants as const
+
+from oracle import born_oracle as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _plane_wave(kind, dtype=np.complex64):
+    import macromax_amd as mm
+    wavelength = 500e-9
+    k0 = 2 * np.pi / wavelength
+    n = 1024
+    pitch = wavelength / 16
+    bt = 10e-6
+    x = pitch * np.arange(n) - bt
+    f = np.maximum(0, np.maximum((x[0] + bt) - x, x - (x[-1] - bt)) / bt)
+    eps = (1 + 1j * 0.1 * f) ** 2
+    J = np.zeros(n)
+    J[np.argmin(np.abs(x))] = 1.0
+    if kind == 'scalar':
+        Jc = J.astype(dtype)
+    else:
+        Jc = (np.array([0, 1, 0])[:, None] * J[None]).astype(dtype)
+        if kind == 'anisotropic':
+            eps = eps * np.eye(3)[..., None]
+            eps[:, :, f <= 0] = np.array([[1.1, 0, 0.3], [0, 1, 0], [0.3, 0, 1.1]])[..., None]
+    sol = mm.solve(x, vacuum_wavelength=wavelength, current_density=Jc, epsilon=eps,
+                   callback=lambda s: s.residue > 1e-6 and s.iteration < 1e4)
+    ref = const.mu_0 * pitch / 2 * const.c * np.exp(1j * k0 * np.abs(x)) * np.exp(-0.1 * k0 * bt * f ** 2 / 2)
+    ref = ref[None] if kind == 'scalar' else np.array([0, 1, 0])[:, None] * ref
+    sel = (wavelength * 10 < x) & (x < x[-1] - bt - wavelength * 10)
+    return sol, ref, sel
+
+
+@pytest.mark.parametrize('kind,max_iter', [('scalar', 70), ('vectorial', 70), ('anisotropic', 85)])
+def test_analytic_plane_wave(kind, max_iter):
+    """tests/test_solution.py:91-358 on the HIP backend (complex64)."""
+    print(f'plane wave {kind}', flush=True)
+    sol, ref, sel = _plane_wave(kind)
+    assert sol.residue < 1e-6
+    assert sol.iteration <= max_iter
+    E = sol.E
+    assert E.dtype == np.complex64
+    err = E - ref
+    npt.assert_almost_equal(np.sqrt(np.mean(np.abs(err[:, sel]) ** 2)) / np.sqrt(np.mean(np.abs(E[:, sel]) ** 2)), 0, decimal=3)
+    npt.assert_almost_equal(np.sqrt(np.mean(np.abs(err) ** 2)) / np.sqrt(np.mean(np.abs(E) ** 2)), 0, decimal=2)
+    if kind != 'scalar':
+        B = sol.B
+        H = sol.H
+        npt.assert_array_almost_equal(B[2, sel] * const.c, E[1, sel], decimal=4)
+        npt.assert_array_almost_equal(B[2, sel] / const.mu_0, H[2, sel], decimal=10)
+        if kind == 'vectorial':
+            npt.assert_array_equal(E[[0, 2]], 0)
+
+
+def test_refractive_index_update_and_resolve():
+    """tests/test_solution.py:246-271: setting refractive_index rebuilds the preconditioner; the modified medium
+    converges, and restoring the index and re-solving recovers the analytic plane wave."""
+    sol, ref, sel = _plane_wave('vectorial')
+    n_before = sol.refractive_index
+    npt.assert_allclose(sol.epsilon, n_before ** 2, rtol=1e-5)
+    alpha_before = sol.alpha
+    x = np.asarray(sol.grid[0]).ravel()
+    slab = (500e-9 <= x) & (x <= 1000e-9)
+    n_new = n_before.copy()
+    n_new[..., slab] = 1.5
+    sol.refractive_index = n_new
+    npt.assert_allclose(sol.epsilon[..., slab], 2.25, rtol=1e-6)
+    assert sol.alpha != alpha_before
+    sol.solve(lambda s: s.residue > 1e-6 and s.iteration < 1e4)
+    assert sol.residue < 1e-6
+    sol.refractive_index = n_before
+    sol.solve(lambda s: s.residue > 1e-6 and s.iteration < 1e4)
+    assert sol.residue < 1e-6
+    E = sol.E
+    err = E - ref
+    npt.assert_almost_equal(np.sqrt(np.mean(np.abs(err[:, sel]) ** 2)) / np.sqrt(np.mean(np.abs(E[:, sel]) ** 2)), 0, decimal=3)
+
+
+@pytest.fixture()
+def sol3d():
+    """tests/test_solution.py:12-32: 50x100x200 vectorial, complex128, absorbing permittivity along axis 0."""
+    import macromax_amd as mm
+    wavelength = 4
+    grid = mm.Grid(shape=[50, 100, 200], step=wavelength / 4)
+    J = np.zeros([3, *grid.shape])
+    bt = 5 * wavelength
+    c = np.asarray(J.shape) // 2
+    J[:, c[1], c[2], c[3]] = [0.0, 1.0, 0.0]
+    d = np.maximum(0.0, np.maximum(grid[0].ravel()[0] + bt - grid[0], grid[0].ravel()[-1] - bt - grid[0]) / bt)
+    return mm.Solution(grid, vacuum_wavelength=wavelength, epsilon=1.0 + 0.2j * d, current_density=J,
+                       dtype=np.complex128)
+
+
+def test_iteration_counter(sol3d):
+    """tests/test_solution.py:46-57."""
+    assert sol3d.iteration == 0
+    it = sol3d.__iter__()
+    next(it)
+    assert sol3d.iteration == 1
+    sol3d.solve(lambda s: s.iteration < 5)
+    assert sol3d.iteration == 5
+    sol3d.iteration = 1
+    next(it)
+    assert sol3d.iteration == 2
+    next(sol3d.__iter__())
+    assert sol3d.iteration == 1
+
+
+def test_update_norm_and_residue(sol3d):
+    """tests/test_solution.py:59-77."""
+    E0 = sol3d.E.copy()
+    next(sol3d.__iter__())
+    E1 = sol3d.E.copy()
+    npt.assert_almost_equal(sol3d.previous_update_norm, np.sqrt(np.sum(np.abs(E1 - E0) ** 2)))
+    npt.assert_almost_equal(sol3d.residue, np.linalg.norm((E1 - E0).ravel()) / np.linalg.norm(E1.ravel()))
+    next(sol3d.__iter__())
+    E2 = sol3d.E.copy()
+    npt.assert_almost_equal(sol3d.previous_update_norm, np.sqrt(np.sum(np.abs(E2 - E1) ** 2)))
+    npt.assert_almost_equal(sol3d.residue, np.linalg.norm((E2 - E1).ravel()) / np.linalg.norm(E2.ravel()))
+
+
+def test_set_E_and_resolve(sol3d):
+    """tests/test_solution.py:79-89."""
+    E0 = sol3d.E
+    sol3d.E = E0
+    npt.assert_almost_equal(sol3d.E, E0)
+    sol3d.solve(lambda s: s.iteration < 10)
+    E_sol = sol3d.E
+    sol3d.E = E0
+    npt.assert_almost_equal(sol3d.E, E0)
+    sol3d.solve(lambda s: s.iteration < 10)
+    npt.assert_almost_equal(sol3d.E, E_sol)
+
+
+def _oracle_from_solution(sol):
+    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    s = sol.source_normalized.cpu().numpy().astype(np.complex128)
+    E = sol.field_array.cpu().numpy().astype(np.complex128)
+    k = ob.k_vectors(tuple(sol.grid.shape), np.asarray(sol.grid.step) * sol.wavenumber)
+    return ob.BornOracle(chi, s, sol.alpha, sol.beta, k, sol.vectorial, E0=E)
+
+
+@pytest.mark.parametrize('case', ['aniso3d', 'iso3d', 'scalar2d', 'vec2d_aniso'])
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128])
+def test_trajectory_matches_oracle(case, dtype):
+    """Per-iteration field E_k of the HIP Solution against the complex128 oracle, same alpha and inputs."""
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print(f'trajectory {case} {np.dtype(dtype).name}', flush=True)
+    rng = np.random.default_rng(7)
+    if case.endswith('3d'):
+        shape = (24, 20, 32)
+    else:
+        shape = (48, 40)
+    grid = mm.Grid(shape=list(shape), step=0.25e-6)
+    nd = len(shape)
+    n = 1.0 + 0.5 * rng.random(shape)
+    if case in ('aniso3d', 'vec2d_aniso'):
+        a = rng.standard_normal((3, *shape))
+        a /= np.linalg.norm(a, axis=0)
+        eps = 1.5 ** 2 * np.eye(3).reshape(3, 3, *[1] * nd) + (1.65 ** 2 - 1.5 ** 2) * a[:, None] * a[None, :]
+    else:
+        eps = n ** 2
+    C = 1 if case == 'scalar2d' else 3
+    J = np.zeros((C, *shape), dtype=np.complex128)
+    J[0, ..., 5] = 1.0
+    if C == 3:
+        J[1, ..., 5] = 0.5j
+    bound = LinearBound(grid, thickness=2e-6)
+    sol = mm.Solution(grid, vacuum_wavelength=1e-6, epsilon=eps, current_density=J.astype(dtype), bound=bound,
+                      dtype=dtype)
+    o = _oracle_from_solution(sol)
+    tol = 1e-5 if dtype == np.complex64 else 1e-10
+    for it in range(1, 11):
+        sol.step()
+        o.step()
+        Eg = sol.field_array.cpu().numpy()
+        err = np.linalg.norm((Eg - o.E).ravel()) / np.linalg.norm(o.E.ravel())
+        assert err < tol, (it, err)
+    npt.assert_allclose(sol.residue, o.relative_residue, rtol=1e-4 if dtype == np.complex64 else 1e-9)
+
+
+def test_divergence_guard_matches_oracle():
+    """solver.py:1083-1096 on the device path: a too-small alpha_i makes updates grow; they are rejected,
+    alpha_i grows by 1.5 per rejection (applied as chi_eff = a chi + c I), and the accepted trajectory follows
+    the oracle, which rescales chi and s in memory as the reference does."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    rng = np.random.default_rng(3)
+    shape = (32, 24, 20)
+    grid = mm.Grid(shape=list(shape), step=0.25e-6)
+    eps = (1.0 + 0.6 * rng.random(shape)) ** 2
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, :, :, 4] = 1.0
+    sol = mm.Solution(grid, vacuum_wavelength=1e-6, epsilon=eps, current_density=J,
+                      bound=LinearBound(grid, thickness=2e-6), dtype=np.complex128)
+    # force an under-damped preconditioner on both sides
+    small = complex(sol.alpha.real, sol.alpha.imag * 0.02)
+    sol._Solution__update_operators(small)
+    o = _oracle_from_solution(sol)
+    rejected = 0
+    for it in range(40):
+        acc = sol.step()
+        acc_o = o.step()
+        assert acc == acc_o, it
+        rejected += not acc
+        npt.assert_allclose(sol.alpha, o.alpha, rtol=1e-12)
+        Eg = sol.field_array.cpu().numpy()
+        assert np.linalg.norm((Eg - o.E).ravel()) <= 1e-9 * max(np.linalg.norm(o.E.ravel()), 1e-300)
+    assert rejected >= 1
