@@ -100,7 +100,9 @@ def main():
     ap.add_argument('--grid', type=int, default=256)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--residual', type=float, default=1e-5)
-    ap.add_argument('--max-iter', type=int, default=2000)
+    ap.add_argument("--max-iter", type=int, default=10000)
+    ap.add_argument('--copy-probe-mb', type=int, default=0,
+                    help='also run one device-to-device copy of this many MiB (calibrates HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus N > 1 is not supported (the multi-GPU '
@@ -124,6 +126,13 @@ def main():
     residue = sol.residue
     print(f'bench: residual {residue:.2e} after {iters_to_res} iterations, {ttr:.3f} s', file=sys.stderr, flush=True)
 
+    if args.copy_probe_mb:
+        src = torch.ones(args.copy_probe_mb * 2 ** 20 // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f'bench: copy probe of {args.copy_probe_mb} MiB done', file=sys.stderr, flush=True)
+        del src, dst
     for _ in range(args.warmup):
         sol.step()
     stream = torch.cuda.current_stream(dev)

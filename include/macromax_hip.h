@@ -51,6 +51,10 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *   MMX_PARAM_P_GREEN    pencils per tile in the fused Green pass (all components in one tile)
  *   MMX_PARAM_XCD_REMAP  1: remap workgroups so that neighbouring tiles run on one XCD; 0: identity
  *   MMX_PARAM_LDS_BUDGET preferred LDS bytes per workgroup
+ *   MMX_PARAM_BIG_ELEMS  bit mask over pass kinds (bit 0 forward, 1 inverse, 2 prologue+forward, 3 Green,
+ *                        4 inverse+update, 5 single-axis): those passes give each thread twice the elements
+ *   MMX_PARAM_REG_FFT    1 (default): complex64 axes of 64, 128, 256 or 512 points use the register-resident
+ *                        FFT kernels; 0: every axis uses the LDS-staged kernels
  */
 #define MMX_PARAM_G_AXIS 1
 #define MMX_PARAM_P_ROWS 2
@@ -58,6 +62,8 @@ int mmx_plan_destroy(mmx_plan_t plan);
 #define MMX_PARAM_P_GREEN 4
 #define MMX_PARAM_XCD_REMAP 5
 #define MMX_PARAM_LDS_BUDGET 6
+#define MMX_PARAM_BIG_ELEMS 7
+#define MMX_PARAM_REG_FFT 8
 int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
 int mmx_plan_get_param(mmx_plan_t plan, int param, int64_t* value);
 

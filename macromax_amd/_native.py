@@ -20,7 +20,8 @@ EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destro
             'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
 
-PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6}
+PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6,
+          'big_elems': 7, 'reg_fft': 8}
 
 
 class NativeError(RuntimeError):

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB_NAME = 'libmacromax_hip.so'
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ['mmx_capi.hip']
-HEADERS = ['mmx_device.hpp', 'mmx_kernels.hpp']
+HEADERS = ['mmx_device.hpp', 'mmx_kernels.hpp', 'mmx_regfft.hpp', 'mmx_regkernels.hpp']
 ARCH = os.environ.get('MMX_OFFLOAD_ARCH', 'gfx950')
 
 

@@ -8,6 +8,7 @@
 
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
+#include "mmx_regkernels.hpp"
 
 using namespace mmx;
 
@@ -35,6 +36,7 @@ constexpr int kElemsFBig = 16;
 constexpr int kElemsDBig = 8;
 
 struct AxisPlan {
+    int orig;        // original grid axis (0..2)
     int N;
     int nrad;
     int rad[kMaxRadices];
@@ -62,6 +64,8 @@ struct mmx_plan_s {
     int g_axis;
     int p_rows, p_strided, p_green;
     int xcd_remap;
+    int big_elems;   // bit k set: pass kind k uses the larger per-thread element budget
+    int reg_fft;     // 1: power-of-two axes use the register-resident FFT kernels
     size_t lds_budget;
 };
 
@@ -133,10 +137,10 @@ struct TileCfg {
 // plain: (batch, component, outer) folded into nouter, ncomp = 1, `nbatch` counts fields.
 template <typename T>
 int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_t lds_max, TileCfg& t,
-                int p_request = 0, size_t lds_budget = kLdsBudget) {
+                int p_request = 0, size_t lds_budget = kLdsBudget, bool want_big = false) {
     int E = sizeof(T) == 4 ? kElemsF : kElemsD;
     t.big = 0;
-    if ((int64_t)(fused ? ncomp : 1) * ax.N > (int64_t)1024 * E) {
+    if (want_big || (int64_t)(fused ? ncomp : 1) * ax.N > (int64_t)1024 * E) {
         E = sizeof(T) == 4 ? kElemsFBig : kElemsDBig;
         t.big = 1;
     }
@@ -211,6 +215,79 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.nd_data = p->ndim;
     a.inv_ntotal = T(1.0 / double(p->V));
     a.comp_stride = p->V;
+    a.axis_orig = ax.orig;
+    a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
+}
+
+// ---- register-resident kernels (complex64, power-of-two N) ----
+constexpr int reg_elems(int N, int NC) { return NC == 3 ? (N >= 64 ? 8 : 4) : (N >= 256 ? 16 : 8); }
+
+bool reg_supported(int N) { return N == 64 || N == 128 || N == 256 || N == 512; }
+
+int reg_threads_per_pencil(int N, int NC) {
+    switch (N) {
+        case 64: return 64 / (NC == 3 ? reg_elems(64, 3) : reg_elems(64, 1));
+        case 128: return 128 / (NC == 3 ? reg_elems(128, 3) : reg_elems(128, 1));
+        case 256: return 256 / (NC == 3 ? reg_elems(256, 3) : reg_elems(256, 1));
+        default: return 512 / (NC == 3 ? reg_elems(512, 3) : reg_elems(512, 1));
+    }
+}
+
+template <int N, int KIND, int NC>
+void launch_reg(const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
+    constexpr int E = reg_elems(N, NC);
+    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, stream, a);
+}
+
+template <int KIND, int NC>
+int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
+    switch (N) {
+        case 64: launch_reg<64, KIND, NC>(t, a, stream); break;
+        case 128: launch_reg<128, KIND, NC>(t, a, stream); break;
+        case 256: launch_reg<256, KIND, NC>(t, a, stream); break;
+        case 512: launch_reg<512, KIND, NC>(t, a, stream); break;
+        default: return fail(MMX_EINVAL, "internal: register FFT length");
+    }
+    return MMX_OK;
+}
+
+// Tile shape of a register-path pass: P pencils (a power of two) of TP threads each, at most 512 threads.
+int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, int p_request, TileCfg& t) {
+    const int NC = fused ? ncomp : 1;
+    const int TP = reg_threads_per_pencil(ax.N, NC);
+    const bool rows = ax.inner == 1;
+    t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
+    const int64_t mult = fused ? nbatch : 1;
+    t.pad = 0;
+    t.big = 0;
+    int pmax = 512 / TP;
+    if (pmax < 1) return fail(MMX_EUNSUP, "internal: register FFT pencil too long");
+    // defaults from the tile sweep at 128^3 and 256^3: 8 rows of 3 components / 16 rows of 1; 32 columns for
+    // single-component strided passes; 8 columns of 3 components (16 of 1) for the Green pass
+    int want = p_request > 0 ? p_request : (rows ? (NC == 3 ? 8 : 16) : (fused ? (NC == 3 ? 8 : 16) : 32));
+    if (want > pmax) want = pmax;
+    if (rows) {
+        const int64_t total_rows = t.nouter * mult;
+        if (p_request <= 0) {
+            while (want > 1 && total_rows / want < 1024) want >>= 1;  // keep >= ~1024 tiles when possible
+            while (want > 1 && want > t.nouter) want >>= 1;
+        }
+        t.P = pow2floor(want);
+        t.ntile_inner = 1;
+        t.tiles_per_problem = (t.nouter + t.P - 1) / t.P;
+    } else {
+        int P = pow2floor(want);
+        while (P > 1 && P > ax.inner) P >>= 1;
+        if (p_request <= 0)
+            while (P > 2 && t.nouter * mult * ((ax.inner + P - 1) / P) < 512) P >>= 1;
+        t.P = P;
+        t.ntile_inner = (ax.inner + P - 1) / P;
+        t.tiles_per_problem = t.nouter * t.ntile_inner;
+    }
+    t.nth = t.P * TP;
+    t.lds = (size_t)(ax.N + t.P * (ax.N + ax.N / 16)) * sizeof(cx<float>);
+    t.nblocks = t.tiles_per_problem * mult;
+    return MMX_OK;
 }
 
 // Launch one pencil pass (see tile_config for `fused`).
@@ -235,13 +312,51 @@ void launch_kind(const TileCfg& t, const PassArgs<T>& a, hipStream_t stream) {
     else launch_kind_e<T, KIND, E>(t, a, stream);
 }
 
-template <typename T>
-int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
-                hipStream_t stream) {
+int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, int64_t nbatch, bool fused,
+                    int kind, hipStream_t stream) {
     TileCfg t;
     const bool rows = ax.inner == 1;
     const int preq = rows ? p->p_rows : (fused ? p->p_green : p->p_strided);
-    int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t, preq, p->lds_budget);
+    if (!fused) a.ncomp = 1;
+    int rc = reg_tile_config(ax, a.ncomp, fused, nbatch, preq, t);
+    if (rc) return rc;
+    a.xcd_remap = p->xcd_remap;
+    if (!fused) a.bstride = 0;
+    a.nouter = t.nouter;
+    a.P = t.P;
+    a.pad = 0;
+    a.ntile_inner = (int)t.ntile_inner;
+    a.tiles_per_problem = t.tiles_per_problem;
+    a.fntile = make_fdiv64(t.ntile_inner);
+    a.ftpp = make_fdiv64(t.tiles_per_problem);
+    if (t.nblocks <= 0) return MMX_OK;
+    if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
+    const bool three = fused && a.ncomp == 3;
+    switch (kind) {
+        case kFwd: rc = launch_reg_n<kFwd, 1>(ax.N, t, a, stream); break;
+        case kInv: rc = launch_reg_n<kInv, 1>(ax.N, t, a, stream); break;
+        case kProFwd: rc = three ? launch_reg_n<kProFwd, 3>(ax.N, t, a, stream) : launch_reg_n<kProFwd, 1>(ax.N, t, a, stream); break;
+        case kGreen: rc = three ? launch_reg_n<kGreen, 3>(ax.N, t, a, stream) : launch_reg_n<kGreen, 1>(ax.N, t, a, stream); break;
+        case kInvEpi: rc = three ? launch_reg_n<kInvEpi, 3>(ax.N, t, a, stream) : launch_reg_n<kInvEpi, 1>(ax.N, t, a, stream); break;
+        case kAll: rc = three ? launch_reg_n<kAll, 3>(ax.N, t, a, stream) : launch_reg_n<kAll, 1>(ax.N, t, a, stream); break;
+        default: return fail(MMX_EINVAL, "internal: unknown pass kind");
+    }
+    if (rc) return rc;
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+template <typename T>
+int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
+                hipStream_t stream) {
+    if constexpr (sizeof(T) == 4) {
+        if (p->reg_fft && reg_supported(ax.N)) return launch_pass_reg(p, ax, a, nbatch, fused, kind, stream);
+    }
+    TileCfg t;
+    const bool rows = ax.inner == 1;
+    const int preq = rows ? p->p_rows : (fused ? p->p_green : p->p_strided);
+    int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t, preq, p->lds_budget,
+                            (p->big_elems >> kind) & 1);
     if (rc) return rc;
     a.xcd_remap = p->xcd_remap;
     if (!fused) {
@@ -317,7 +432,12 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
     const AxisPlan& inner = p->ax[d - 1];
     // partial sums: one pair per tile of the last pass
     TileCfg tl;
-    int rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl, p->p_rows, p->lds_budget);
+    int rc;
+    if (sizeof(T) == 4 && p->reg_fft && reg_supported(inner.N))
+        rc = reg_tile_config(inner, C, true, nbatch, p->p_rows, tl);
+    else
+        rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl, p->p_rows, p->lds_budget,
+                            (p->big_elems >> (d == 1 ? kAll : kInvEpi)) & 1);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
     rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
@@ -484,8 +604,9 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
             v = 65536;
         p->lds_max = (size_t)v;
     }
-    p->g_axis = 0;
+    p->g_axis = 1;  // Green pass on the middle axis of 3-D grids (clamped to the outermost axis otherwise)
     p->xcd_remap = 0;
+    p->reg_fft = 1;
     p->lds_budget = kLdsBudget;
     // non-trivial axes, outermost first
     p->naxes = 0;
@@ -494,6 +615,7 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
         if (n == 1) continue;
         if (n > kMaxN) { delete p; return fail(MMX_EUNSUP, "axis longer than 4096 samples"); }
         AxisPlan& ax = p->ax[p->naxes++];
+        ax.orig = a;
         ax.N = (int)n;
         if (!factorize(ax.N, ax.rad, ax.nrad)) {
             delete p;
@@ -548,6 +670,8 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
         case MMX_PARAM_P_GREEN: p->p_green = (int)value; break;
         case MMX_PARAM_XCD_REMAP: p->xcd_remap = value ? 1 : 0; break;
         case MMX_PARAM_LDS_BUDGET: p->lds_budget = value ? (size_t)value : kLdsBudget; break;
+        case MMX_PARAM_BIG_ELEMS: p->big_elems = (int)value; break;
+        case MMX_PARAM_REG_FFT: p->reg_fft = value ? 1 : 0; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;
@@ -563,6 +687,8 @@ int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
         case MMX_PARAM_P_GREEN: *value = p->p_green; break;
         case MMX_PARAM_XCD_REMAP: *value = p->xcd_remap; break;
         case MMX_PARAM_LDS_BUDGET: *value = (int64_t)p->lds_budget; break;
+        case MMX_PARAM_BIG_ELEMS: *value = p->big_elems; break;
+        case MMX_PARAM_REG_FFT: *value = p->reg_fft; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;

@@ -71,6 +71,8 @@ struct PassArgs {
     int nd_data;              // grid ndim
     int vectorial;
     T inv_ntotal;
+    int axis_orig;            // original grid axis (0..2) of this pass
+    int kofs_axis;            // offset of that axis' k-vector in kvec
 };
 
 // Green's operator at one k for the C components held in f[] (MacroMax solver.py:566-581 with

@@ -1,0 +1,154 @@
+// Register-resident Stockham FFT for power-of-two axis lengths (compile-time N).
+//
+// A pencil of N points is owned by TP = N/E threads; thread t holds the E points at positions t + m*TP
+// (m < E) in registers. Every stage of radix R (R divides E) runs E/R butterflies per thread on those registers;
+// butterfly q of thread t is Stockham butterfly i = t + q*TP with inputs at i + j*N/R, which are exactly the
+// registers m = q + j*E/R. Between stages the outputs go through LDS once (write at their Stockham output
+// position, read back the new stride-TP set); after the last stage register m holds output t + m*TP, so the
+// first stage reads and the last stage writes HBM directly, with no LDS staging.
+#pragma once
+#include "mmx_device.hpp"
+
+namespace mmx {
+
+template <typename T, int DIR> __device__ __forceinline__ void dft16(cx<T>* a) {
+    // 16 = 4 x 4: DFT4 over n1 (stride 4), twiddle W16^(n2 k1), DFT4 over n2; output k = k1 + 4 k2
+    cx<T> b[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        cx<T> c[4] = {a[n2], a[4 + n2], a[8 + n2], a[12 + n2]};
+        dft4<T, DIR>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) b[4 * k1 + n2] = c[k1];
+    }
+    const T h = T(0.70710678118654752440084436210485);
+    const T c1 = T(0.92387953251128675612818318939679), s1 = T(0.38268343236508977172845998403040);
+    // b[4*k1 + n2] *= W16^(n2*k1), W16^m = cos(2 pi m/16) + DIR i sin(2 pi m/16)
+    auto tw = [&](cx<T> x, T cr, T si) { return cx<T>{x.r * cr - T(DIR) * x.i * si, x.i * cr + T(DIR) * x.r * si}; };
+    b[5] = tw(b[5], c1, s1);      // m = 1
+    b[6] = tw(b[6], h, h);        // m = 2
+    b[7] = tw(b[7], s1, c1);      // m = 3
+    b[9] = tw(b[9], h, h);        // m = 2
+    b[10] = mul_dir_i<T, DIR>(b[10]);  // m = 4
+    b[11] = tw(b[11], -h, h);     // m = 6
+    b[13] = tw(b[13], s1, c1);    // m = 3
+    b[14] = tw(b[14], -h, h);     // m = 6
+    b[15] = tw(b[15], -c1, -s1);  // m = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        cx<T> c[4] = {b[4 * k1], b[4 * k1 + 1], b[4 * k1 + 2], b[4 * k1 + 3]};
+        dft4<T, DIR>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = c[k2];
+    }
+}
+
+template <typename T, int R, int DIR> __device__ __forceinline__ void dft_pow2(cx<T>* a) {
+    if constexpr (R == 2) dft2<T, DIR>(a);
+    else if constexpr (R == 4) dft4<T, DIR>(a);
+    else if constexpr (R == 8) dft8<T, DIR>(a);
+    else dft16<T, DIR>(a);
+}
+
+constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+
+// Radix of stage s for N = 2^n points and E registers per thread: the largest power of two <= min(E, 16)
+// that still divides what is left of N.
+constexpr int stage_radix(int N, int E, int s) {
+    int rem = N, r = 1;
+    for (int q = 0; q <= s; ++q) {
+        r = E < 16 ? E : 16;
+        while (r > rem) r >>= 1;
+        rem /= r;
+    }
+    return r;
+}
+constexpr int stage_prod(int N, int E, int s) {  // product of the radices before stage s
+    int p = 1;
+    for (int q = 0; q < s; ++q) p *= stage_radix(N, E, q);
+    return p;
+}
+constexpr int num_stages(int N, int E) {
+    int s = 0, p = 1;
+    while (p < N) { p *= stage_radix(N, E, s); ++s; }
+    return s;
+}
+
+// LDS position with one pad slot every 16 entries (breaks the power-of-two strides of the exchange).
+__device__ __forceinline__ int padpos(int pos) { return pos + (pos >> 4); }
+constexpr int padded_len(int N) { return N + N / 16; }
+
+// One stage on the registers v[NC][E] of all NC components.
+template <typename T, int N, int E, int S, int DIR, int NC>
+__device__ __forceinline__ void reg_stage(cx<T> (&v)[NC][E], int t, const cx<T>* tw) {
+    constexpr int R = stage_radix(N, E, S);
+    constexpr int P = stage_prod(N, E, S);
+    constexpr int TP = N / E;
+    constexpr int BQ = E / R;
+    constexpr int TWS = N / (P * R);
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) {
+        const int i = t + q * TP;
+        const int k = i & (P - 1);
+        cx<T> w[R];
+        if constexpr (P > 1) {
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                w[j] = tw[j * k * TWS];
+                if (DIR > 0) w[j].i = -w[j].i;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            cx<T> a[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) a[j] = v[c][q + j * BQ];
+            if constexpr (P > 1) {
+#pragma unroll
+                for (int j = 1; j < R; ++j) a[j] = a[j] * w[j];
+            }
+            dft_pow2<T, R, DIR>(a);
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[c][q + j * BQ] = a[j];
+        }
+    }
+}
+
+// Exchange after stage S: outputs to their Stockham positions in LDS, then read back positions t + m*TP.
+// `xb` is this pencil's exchange buffer (padded_len(N) entries, with stride `ps` between entries: 1 for
+// pencil-major buffers, the pencil count for interleaved ones). All NC components go through the buffer one
+// after the other.
+template <typename T, int N, int E, int S, int NC>
+__device__ __forceinline__ void reg_exchange(cx<T> (&v)[NC][E], int t, cx<T>* xb, int ps) {
+    constexpr int R = stage_radix(N, E, S);
+    constexpr int P = stage_prod(N, E, S);
+    constexpr int TP = N / E;
+    constexpr int BQ = E / R;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) {
+            const int i = t + q * TP;
+            const int k = i & (P - 1);
+            const int base = (i - k) * R + k;
+#pragma unroll
+            for (int j = 0; j < R; ++j) xb[padpos(base + j * P) * ps] = v[c][q + j * BQ];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < E; ++m) v[c][m] = xb[padpos(t + m * TP) * ps];
+        __syncthreads();
+    }
+}
+
+// Whole FFT (unnormalised) of the NC pencils held in v, natural order in and out.
+template <typename T, int N, int E, int DIR, int NC, int S = 0>
+__device__ __forceinline__ void reg_fft(cx<T> (&v)[NC][E], int t, const cx<T>* tw, cx<T>* xb, int ps) {
+    reg_stage<T, N, E, S, DIR, NC>(v, t, tw);
+    if constexpr (S + 1 < num_stages(N, E)) {
+        reg_exchange<T, N, E, S, NC>(v, t, xb, ps);
+        reg_fft<T, N, E, DIR, NC, S + 1>(v, t, tw, xb, ps);
+    }
+}
+
+}  // namespace mmx
