@@ -98,6 +98,26 @@ int mmx_born_step(mmx_plan_t plan, const void* E, void* Eout, const void* s, con
                   int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
                   double* norms, void* stream);
 
+/*
+ * The same iteration split so that consecutive iterations share one pass over the field (the last pass of
+ * iteration n also computes the first pass of iteration n+1, reading chi, E and s once). W is a work buffer of the
+ * size of E, distinct from E and Eout.
+ *   mmx_born_prologue: W = FFT along the innermost axis of (chi_eff E + s_eff); call it before the first
+ *                      mmx_born_iterate, and again whenever E, s, chi or coef changed outside mmx_born_iterate
+ *                      (e.g. after a rejected update).
+ *   mmx_born_iterate:  completes the iteration begun in W: Eout = E + d, norms as for mmx_born_step, and
+ *                      W = FFT along the innermost axis of (chi_eff Eout + s_eff), the prologue of the next
+ *                      iteration assuming Eout is accepted. If the caller rejects Eout it must re-run
+ *                      mmx_born_prologue on E (with the new coef) before the next mmx_born_iterate.
+ * Replaces Solution.__iter__ (solver.py:1064-1098) for callers that iterate many times.
+ */
+int mmx_born_prologue(mmx_plan_t plan, const void* E, void* W, const void* s, const void* chi, int ncomp,
+                      int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                      void* stream);
+int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const void* s, const void* chi, int ncomp,
+                     int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                     double* norms, void* stream);
+
 /* out[0] = sum |x|^2 over n complex elements, FP64 accumulation (backend/__init__.py:872-874, torch.py:316-317). */
 int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stream);
 

@@ -17,7 +17,8 @@ MMX_C64 = 0
 MMX_C128 = 1
 
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
-            'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_norm2', 'mmx_chi_mul',
+            'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue', 'mmx_born_iterate',
+            'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
 
 PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6,
@@ -50,6 +51,10 @@ def _declare(L):
     L.mmx_fft.argtypes = [c_p, c_p, c_p, c_i64, c_int, c_p]
     L.mmx_born_step.restype = c_int
     L.mmx_born_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
+    L.mmx_born_prologue.restype = c_int
+    L.mmx_born_prologue.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p]
+    L.mmx_born_iterate.restype = c_int
+    L.mmx_born_iterate.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
     L.mmx_norm2.restype = c_int
     L.mmx_norm2.argtypes = [c_p, c_p, c_i64, c_p, c_p]
     L.mmx_chi_mul.restype = c_int

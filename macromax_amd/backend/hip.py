@@ -142,6 +142,43 @@ class Plan:
                                               _stream_ptr(self.device)), 'mmx_born_step')
         return norms
 
+    def _born_checks(self, E, others, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride):
+        for t, w in ((E, 'E'), (s, 's'), (chi, 'chi')) + tuple(others):
+            self._check(t, w)
+        if coef.dtype != self.real_dtype or coef.device != self.device or coef.numel() < 6 * nbatch:
+            raise ValueError('coef must hold 6 reals of the plan precision per problem')
+        if E.numel() != nbatch * ncomp * self.V:
+            raise ValueError('E size mismatch')
+        for t, w in others:
+            if t.numel() != E.numel():
+                raise ValueError(f'{w} size mismatch')
+        chi_size = {1: 1, 2: 9, 3: 6}[int(chi_kind)] * self.V
+        if chi.numel() != (chi_size if chi_bstride == 0 else chi_size * nbatch):
+            raise ValueError('chi size mismatch')
+        if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
+            raise ValueError('s size mismatch')
+
+    def born_prologue(self, E, W, s, chi, ncomp, chi_kind, coef, nbatch=1, s_bstride=0, chi_bstride=0):
+        """W = FFT along the innermost axis of chi_eff E + s_eff (``mmx_born_prologue``)."""
+        self._born_checks(E, ((W, 'W'),), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride)
+        _native.check(self._lib.mmx_born_prologue(self.handle, E.data_ptr(), W.data_ptr(), s.data_ptr(),
+                                                  chi.data_ptr(), int(ncomp), int(chi_kind), coef.data_ptr(),
+                                                  int(nbatch), int(s_bstride), int(chi_bstride),
+                                                  _stream_ptr(self.device)), 'mmx_born_prologue')
+        return W
+
+    def born_iterate(self, E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch=1, s_bstride=0, chi_bstride=0):
+        """Finish the iteration begun in W (``mmx_born_iterate``): Eout = E + d, norms, W = next prologue."""
+        self._born_checks(E, ((Eout, 'Eout'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
+                          chi_bstride)
+        if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
+            raise ValueError('norms must be a float64 device tensor of 2*nbatch elements')
+        _native.check(self._lib.mmx_born_iterate(self.handle, E.data_ptr(), Eout.data_ptr(), W.data_ptr(),
+                                                 s.data_ptr(), chi.data_ptr(), int(ncomp), int(chi_kind),
+                                                 coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
+                                                 norms.data_ptr(), _stream_ptr(self.device)), 'mmx_born_iterate')
+        return norms
+
     def norm2(self, x, out):
         self._check(x, 'norm2 input')
         _native.check(self._lib.mmx_norm2(self.handle, x.data_ptr(), x.numel(), out.data_ptr(),
@@ -475,6 +512,17 @@ class BackEndHip(BackEnd):
         """d = chi [G(chi E + s) - E]; Eout = E + d; norms = [sum |d|^2, sum |Eout|^2] (FP64, on the device)."""
         ncomp = E.shape[-self.grid.ndim - 2] if E.ndim >= self.grid.ndim + 2 else 1
         return self.__plan.born_step(E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms)
+
+    def born_prologue(self, E, W, s, chi, chi_kind: int, coef, nbatch: int = 1, s_bstride: int = 0,
+                      chi_bstride: int = 0):
+        ncomp = E.shape[-self.grid.ndim - 2]
+        return self.__plan.born_prologue(E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride)
+
+    def born_iterate(self, E, Eout, W, s, chi, chi_kind: int, coef, norms, nbatch: int = 1, s_bstride: int = 0,
+                     chi_bstride: int = 0):
+        ncomp = E.shape[-self.grid.ndim - 2]
+        return self.__plan.born_iterate(E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride,
+                                        chi_bstride)
 
     def __str__(self) -> str:
         return f'BackEndHip(vectorial={self.vectorial}, {self.grid}, dtype={self.numpy_dtype}, device={self.__device})'

@@ -220,41 +220,72 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 }
 
 // ---- register-resident kernels (complex64, power-of-two N) ----
-constexpr int reg_elems(int N, int NC) { return NC == 3 ? (N >= 64 ? 8 : 4) : (N >= 256 ? 16 : 8); }
-
-bool reg_supported(int N) { return N == 64 || N == 128 || N == 256 || N == 512; }
-
-int reg_threads_per_pencil(int N, int NC) {
-    switch (N) {
-        case 64: return 64 / (NC == 3 ? reg_elems(64, 3) : reg_elems(64, 1));
-        case 128: return 128 / (NC == 3 ? reg_elems(128, 3) : reg_elems(128, 1));
-        case 256: return 256 / (NC == 3 ? reg_elems(256, 3) : reg_elems(256, 1));
-        default: return 512 / (NC == 3 ? reg_elems(512, 3) : reg_elems(512, 1));
-    }
+// points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels
+constexpr int reg_elems(int N, int NC, int KIND) {
+    return (KIND == kEpiPro || KIND == kGreenEpiPro) ? 8 : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
 }
 
-template <int N, int KIND, int NC>
+// Power-of-two lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the
+// strided extent N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
+bool reg_supported(const AxisPlan& ax) {
+    const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512;
+    return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
+}
+
+int reg_threads_per_pencil(int N, int NC, int kind) {
+    const bool fusedlast = (kind == kEpiPro || kind == kGreenEpiPro);
+    return N / reg_elems(N, NC, fusedlast ? kEpiPro : kFwd);
+}
+
+template <int N, int KIND, int NC, int CK>
 void launch_reg(const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
-    constexpr int E = reg_elems(N, NC);
-    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, stream, a);
+    constexpr int E = reg_elems(N, NC, KIND);
+    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds,
+                       stream, a);
 }
 
-template <int KIND, int NC>
+template <int KIND, int NC, int CK>
 int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
     switch (N) {
-        case 64: launch_reg<64, KIND, NC>(t, a, stream); break;
-        case 128: launch_reg<128, KIND, NC>(t, a, stream); break;
-        case 256: launch_reg<256, KIND, NC>(t, a, stream); break;
-        case 512: launch_reg<512, KIND, NC>(t, a, stream); break;
+        case 64: launch_reg<64, KIND, NC, CK>(t, a, stream); break;
+        case 128: launch_reg<128, KIND, NC, CK>(t, a, stream); break;
+        case 256: launch_reg<256, KIND, NC, CK>(t, a, stream); break;
+        case 512: launch_reg<512, KIND, NC, CK>(t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: register FFT length");
     }
     return MMX_OK;
 }
 
+constexpr bool kind_applies_chi(int kind) {
+    return kind == kProFwd || kind == kInvEpi || kind == kAll || kind == kEpiPro || kind == kGreenEpiPro;
+}
+
+// Instantiation for the component count and (for passes that apply chi) the chi kind of the launch.
+template <int KIND>
+int launch_reg_kind(int N, bool three, int chi_kind, const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
+    if constexpr (KIND == kFwd || KIND == kInv) {  // single-component passes
+        if (three) return fail(MMX_EINVAL, "internal: plain pass with three components");
+        return launch_reg_n<KIND, 1, 0>(N, t, a, stream);
+    } else if constexpr (!kind_applies_chi(KIND)) {
+        return three ? launch_reg_n<KIND, 3, 0>(N, t, a, stream) : launch_reg_n<KIND, 1, 0>(N, t, a, stream);
+    } else {
+        if (!three) {
+            if (chi_kind != 1) return fail(MMX_EINVAL, "internal: 3x3 chi with one component");
+            return launch_reg_n<KIND, 1, 1>(N, t, a, stream);
+        }
+        switch (chi_kind) {
+            case 1: return launch_reg_n<KIND, 3, 1>(N, t, a, stream);
+            case 2: return launch_reg_n<KIND, 3, 2>(N, t, a, stream);
+            case 3: return launch_reg_n<KIND, 3, 3>(N, t, a, stream);
+            default: return fail(MMX_EINVAL, "internal: chi kind");
+        }
+    }
+}
+
 // Tile shape of a register-path pass: P pencils (a power of two) of TP threads each, at most 512 threads.
-int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, int p_request, TileCfg& t) {
+int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, int p_request, int kind, TileCfg& t) {
     const int NC = fused ? ncomp : 1;
-    const int TP = reg_threads_per_pencil(ax.N, NC);
+    const int TP = reg_threads_per_pencil(ax.N, NC, kind);
     const bool rows = ax.inner == 1;
     t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
     const int64_t mult = fused ? nbatch : 1;
@@ -318,7 +349,7 @@ int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, 
     const bool rows = ax.inner == 1;
     const int preq = rows ? p->p_rows : (fused ? p->p_green : p->p_strided);
     if (!fused) a.ncomp = 1;
-    int rc = reg_tile_config(ax, a.ncomp, fused, nbatch, preq, t);
+    int rc = reg_tile_config(ax, a.ncomp, fused, nbatch, preq, kind, t);
     if (rc) return rc;
     a.xcd_remap = p->xcd_remap;
     if (!fused) a.bstride = 0;
@@ -332,13 +363,16 @@ int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, 
     if (t.nblocks <= 0) return MMX_OK;
     if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
     const bool three = fused && a.ncomp == 3;
+    const int ck = a.chi_kind;
     switch (kind) {
-        case kFwd: rc = launch_reg_n<kFwd, 1>(ax.N, t, a, stream); break;
-        case kInv: rc = launch_reg_n<kInv, 1>(ax.N, t, a, stream); break;
-        case kProFwd: rc = three ? launch_reg_n<kProFwd, 3>(ax.N, t, a, stream) : launch_reg_n<kProFwd, 1>(ax.N, t, a, stream); break;
-        case kGreen: rc = three ? launch_reg_n<kGreen, 3>(ax.N, t, a, stream) : launch_reg_n<kGreen, 1>(ax.N, t, a, stream); break;
-        case kInvEpi: rc = three ? launch_reg_n<kInvEpi, 3>(ax.N, t, a, stream) : launch_reg_n<kInvEpi, 1>(ax.N, t, a, stream); break;
-        case kAll: rc = three ? launch_reg_n<kAll, 3>(ax.N, t, a, stream) : launch_reg_n<kAll, 1>(ax.N, t, a, stream); break;
+        case kFwd: rc = launch_reg_kind<kFwd>(ax.N, three, ck, t, a, stream); break;
+        case kInv: rc = launch_reg_kind<kInv>(ax.N, three, ck, t, a, stream); break;
+        case kProFwd: rc = launch_reg_kind<kProFwd>(ax.N, three, ck, t, a, stream); break;
+        case kGreen: rc = launch_reg_kind<kGreen>(ax.N, three, ck, t, a, stream); break;
+        case kInvEpi: rc = launch_reg_kind<kInvEpi>(ax.N, three, ck, t, a, stream); break;
+        case kAll: rc = launch_reg_kind<kAll>(ax.N, three, ck, t, a, stream); break;
+        case kEpiPro: rc = launch_reg_kind<kEpiPro>(ax.N, three, ck, t, a, stream); break;
+        case kGreenEpiPro: rc = launch_reg_kind<kGreenEpiPro>(ax.N, three, ck, t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: unknown pass kind");
     }
     if (rc) return rc;
@@ -350,7 +384,7 @@ template <typename T>
 int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
                 hipStream_t stream) {
     if constexpr (sizeof(T) == 4) {
-        if (p->reg_fft && reg_supported(ax.N)) return launch_pass_reg(p, ax, a, nbatch, fused, kind, stream);
+        if (p->reg_fft && reg_supported(ax)) return launch_pass_reg(p, ax, a, nbatch, fused, kind, stream);
     }
     TileCfg t;
     const bool rows = ax.inner == 1;
@@ -382,6 +416,8 @@ int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t 
         case kGreen: launch_kind<T, kGreen>(t, a, stream); break;
         case kInvEpi: launch_kind<T, kInvEpi>(t, a, stream); break;
         case kAll: launch_kind<T, kAll>(t, a, stream); break;
+        case kEpiPro: launch_kind<T, kEpiPro>(t, a, stream); break;
+        case kGreenEpiPro: launch_kind<T, kGreenEpiPro>(t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: unknown pass kind");
     }
     MMX_HIP_CHECK(hipGetLastError());
@@ -419,6 +455,136 @@ int fft_impl(mmx_plan_s* p, const void* in, void* out, int64_t nbatch, int direc
     return MMX_OK;
 }
 
+// Tile configuration of the pass that writes the norm partials (same choice as launch_pass makes).
+template <typename T>
+int last_pass_tiles(const mmx_plan_s* p, const AxisPlan& inner, int C, int64_t nbatch, int kind, TileCfg& tl) {
+    if (sizeof(T) == 4 && p->reg_fft && reg_supported(inner))
+        return reg_tile_config(inner, C, true, nbatch, p->p_rows, kind, tl);
+    return tile_config<T>(inner, C, true, nbatch, p->lds_max, tl, p->p_rows, p->lds_budget, (p->big_elems >> kind) & 1);
+}
+
+// norms[2b + {0,1}] = sums of the last pass's per-tile partials of problem b, in tile order.
+int reduce_norms(const double* partials, int64_t tiles, int64_t nbatch, double* norms, hipStream_t stream) {
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nbatch), dim3(512), 0, stream, partials, tiles, norms);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+template <typename T>
+int born_check(const mmx_plan_s* p, int C, int chi_kind) {
+    if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
+    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
+    if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
+    if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
+    return MMX_OK;
+}
+
+template <typename T>
+PassArgs<T> born_base(const mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* chi, int C,
+                      int chi_kind, const void* coef, int64_t s_bstride, int64_t chi_bstride, double* partials,
+                      double* norms = nullptr) {
+    PassArgs<T> base = blank_args<T>();
+    base.norms = norms;
+    base.C = C;
+    base.chi_kind = chi_kind;
+    base.E = reinterpret_cast<const cx<T>*>(E);
+    base.chi = reinterpret_cast<const cx<T>*>(chi);
+    base.s = reinterpret_cast<const cx<T>*>(s);
+    base.E_bstride = (int64_t)C * p->V;
+    base.s_bstride = s_bstride;
+    base.chi_bstride = chi_bstride;
+    base.Eout = reinterpret_cast<cx<T>*>(Eout);
+    base.partials = partials;
+    base.coef = reinterpret_cast<const Coef<T>*>(coef);
+    base.vectorial = (C == 3);
+    return base;
+}
+
+// W = forward FFT along the innermost axis of chi_eff E + s_eff (the first pass of an iteration).
+template <typename T>
+int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const void* chi, int C, int chi_kind,
+                  const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, hipStream_t stream) {
+    int rc = born_check<T>(p, C, chi_kind);
+    if (rc) return rc;
+    if (nbatch <= 0) return MMX_OK;
+    const AxisPlan& inner = p->ax[p->naxes - 1];
+    PassArgs<T> a = born_base<T>(p, E, nullptr, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, nullptr);
+    fill_geometry<T>(p, inner, a);
+    a.ncomp = C;
+    a.bstride = (int64_t)C * p->V;
+    a.src = reinterpret_cast<cx<T>*>(W);
+    a.dst = reinterpret_cast<cx<T>*>(W);
+    return launch_pass<T>(p, inner, a, nbatch, true, kProFwd, stream);
+}
+
+// One iteration from W (first pass already applied to E): remaining forward passes, Green, inverse passes, then
+// the fused last pass: Eout = E + chi_eff (W - E), norms, and W <- first pass of the next iteration on Eout.
+template <typename T>
+int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void* s, const void* chi, int C,
+                 int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                 double* norms, hipStream_t stream) {
+    int rc = born_check<T>(p, C, chi_kind);
+    if (rc) return rc;
+    if (nbatch <= 0) return MMX_OK;
+    const int d = p->naxes;
+    const AxisPlan& inner = p->ax[d - 1];
+    const int last_kind = d == 1 ? kGreenEpiPro : kEpiPro;
+    TileCfg tl;
+    rc = last_pass_tiles<T>(p, inner, C, nbatch, last_kind, tl);
+    if (rc) return rc;
+    const int64_t tiles_last = tl.tiles_per_problem;
+    rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
+    if (rc) return rc;
+    double* partials = reinterpret_cast<double*>(p->scratch);
+    PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
+    cx<T>* W = reinterpret_cast<cx<T>*>(Wv);
+    if (d >= 2) {
+        int g = p->g_axis;
+        if (g < 0) g = 0;
+        if (g > d - 2) g = d - 2;
+        int others[3], nothers = 0;
+        for (int q = d - 2; q >= 0; --q)
+            if (q != g) others[nothers++] = q;
+        for (int q = 0; q < nothers; ++q) {
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[others[q]], a);
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kFwd, stream);
+            if (rc) return rc;
+        }
+        {
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[g], a);
+            a.ncomp = C;
+            a.bstride = (int64_t)C * p->V;
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[g], a, nbatch, true, kGreen, stream);
+            if (rc) return rc;
+        }
+        for (int q = nothers - 1; q >= 0; --q) {
+            PassArgs<T> a = base;
+            fill_geometry<T>(p, p->ax[others[q]], a);
+            a.src = W;
+            a.dst = W;
+            rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kInv, stream);
+            if (rc) return rc;
+        }
+    }
+    {
+        PassArgs<T> a = base;
+        fill_geometry<T>(p, inner, a);
+        a.ncomp = C;
+        a.bstride = (int64_t)C * p->V;
+        a.src = W;
+        a.dst = W;
+        rc = launch_pass<T>(p, inner, a, nbatch, true, last_kind, stream);
+        if (rc) return rc;
+    }
+    return reduce_norms(partials, tiles_last, nbatch, norms, stream);
+}
+
 template <typename T>
 int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* chi, int C, int chi_kind,
               const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
@@ -432,31 +598,13 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
     const AxisPlan& inner = p->ax[d - 1];
     // partial sums: one pair per tile of the last pass
     TileCfg tl;
-    int rc;
-    if (sizeof(T) == 4 && p->reg_fft && reg_supported(inner.N))
-        rc = reg_tile_config(inner, C, true, nbatch, p->p_rows, tl);
-    else
-        rc = tile_config<T>(inner, C, true, nbatch, p->lds_max, tl, p->p_rows, p->lds_budget,
-                            (p->big_elems >> (d == 1 ? kAll : kInvEpi)) & 1);
+    int rc = last_pass_tiles<T>(p, inner, C, nbatch, d == 1 ? kAll : kInvEpi, tl);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
     rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
     if (rc) return rc;
     double* partials = reinterpret_cast<double*>(p->scratch);
-
-    PassArgs<T> base = blank_args<T>();
-    base.C = C;
-    base.chi_kind = chi_kind;
-    base.E = reinterpret_cast<const cx<T>*>(E);
-    base.chi = reinterpret_cast<const cx<T>*>(chi);
-    base.s = reinterpret_cast<const cx<T>*>(s);
-    base.E_bstride = (int64_t)C * p->V;
-    base.s_bstride = s_bstride;
-    base.chi_bstride = chi_bstride;
-    base.Eout = reinterpret_cast<cx<T>*>(Eout);
-    base.partials = partials;
-    base.coef = reinterpret_cast<const Coef<T>*>(coef);
-    base.vectorial = (C == 3);
+    PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
     cx<T>* W = reinterpret_cast<cx<T>*>(Eout);
 
     if (d == 1) {
@@ -526,9 +674,7 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
             if (rc) return rc;
         }
     }
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nbatch), dim3(256), 0, stream, partials, tiles_last, norms);
-    MMX_HIP_CHECK(hipGetLastError());
-    return MMX_OK;
+    return reduce_norms(partials, tiles_last, nbatch, norms, stream);
 }
 
 template <typename T>
@@ -713,6 +859,29 @@ int mmx_born_step(mmx_plan_t p, const void* E, void* Eout, const void* s, const 
     return p->dtype == MMX_C64
                ? born_impl<float>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
                : born_impl<double>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
+}
+
+int mmx_born_prologue(mmx_plan_t p, const void* E, void* W, const void* s, const void* chi, int ncomp, int chi_kind,
+                      const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, void* stream) {
+    g_last_error.clear();
+    if (!p || !E || !W || !s || !chi || !coef) return fail(MMX_EINVAL, "null argument");
+    if (E == W) return fail(MMX_EINVAL, "E and W must be different buffers");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64
+               ? prologue_impl<float>(p, E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, st)
+               : prologue_impl<double>(p, E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, st);
+}
+
+int mmx_born_iterate(mmx_plan_t p, const void* E, void* Eout, void* W, const void* s, const void* chi, int ncomp,
+                     int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                     double* norms, void* stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !W || !s || !chi || !coef || !norms) return fail(MMX_EINVAL, "null argument");
+    if (E == Eout || E == W || Eout == W) return fail(MMX_EINVAL, "E, Eout and W must be three different buffers");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64
+               ? iterate_impl<float>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
+               : iterate_impl<double>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
 }
 
 int mmx_norm2(mmx_plan_t p, const void* x, int64_t n, double* out, void* stream) {

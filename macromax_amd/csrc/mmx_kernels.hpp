@@ -63,6 +63,7 @@ struct PassArgs {
     int64_t E_bstride, chi_bstride, s_bstride;
     cx<T>* Eout;
     double* partials;         // 2 per (problem, tile)
+    double* norms;            // per problem: {sum |d|^2, sum |Eout|^2}
     const Coef<T>* coef;      // one per problem
     // Green's function
     int64_t n0, n1, n2;       // grid shape padded with trailing 1s
@@ -119,35 +120,74 @@ __device__ __forceinline__ void unravel3(int64_t v, const PassArgs<T>& a, int64_
     i1 = t - i0 * a.n1;
 }
 
-// chi_eff applied to the C-vector x at voxel v of problem b. chi_kind 1: one value per voxel; 2: 3x3 row-major
-// [9, V]; 3: complex-symmetric 3x3 packed as [xx, xy, xz, yy, yz, zz] ([6, V]).
+// chi at one voxel, loaded once and applied as often as needed. chi_kind 1: one value per voxel; 2: 3x3
+// row-major [9, V]; 3: complex-symmetric 3x3 packed as [xx, xy, xz, yy, yz, zz] ([6, V]).
+template <typename T>
+struct ChiVox {
+    cx<T> m0, m1, m2, m3, m4, m5, m6, m7, m8;  // named (not an array): every access stays in registers
+};
+
+template <typename T>
+__device__ __forceinline__ void chi_load(const PassArgs<T>& a, const cx<T>* chi_b, int64_t v, ChiVox<T>& q) {
+    const int64_t V = a.comp_stride;
+    q.m0 = chi_b[v];
+    if (a.chi_kind >= 2) {
+        q.m1 = chi_b[V + v];
+        q.m2 = chi_b[2 * V + v];
+        q.m3 = chi_b[3 * V + v];
+        q.m4 = chi_b[4 * V + v];
+        q.m5 = chi_b[5 * V + v];
+        if (a.chi_kind == 2) {
+            q.m6 = chi_b[6 * V + v];
+            q.m7 = chi_b[7 * V + v];
+            q.m8 = chi_b[8 * V + v];
+        }
+    }
+}
+
+// y = chi_eff x = a chi x + c x
+template <typename T>
+__device__ __forceinline__ void chi_mulv(const PassArgs<T>& a, const ChiVox<T>& q, const cx<T>* x, cx<T>* y,
+                                        const Coef<T>& cf) {
+    const cx<T> cc{cf.c_r, cf.c_i};
+    if (a.chi_kind == 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = scal(q.m0 * x[c], cf.a) + cc * x[c];
+        return;
+    }
+    cx<T> y0, y1, y2;
+    if (a.chi_kind == 2) {
+        y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
+        y1 = q.m3 * x[0] + q.m4 * x[1] + q.m5 * x[2];
+        y2 = q.m6 * x[0] + q.m7 * x[1] + q.m8 * x[2];
+    } else {  // packed symmetric {xx, xy, xz, yy, yz, zz}
+        y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
+        y1 = q.m1 * x[0] + q.m3 * x[1] + q.m4 * x[2];
+        y2 = q.m2 * x[0] + q.m4 * x[1] + q.m5 * x[2];
+    }
+    y[0] = scal(y0, cf.a) + cc * x[0];
+    y[1] = scal(y1, cf.a) + cc * x[1];
+    y[2] = scal(y2, cf.a) + cc * x[2];
+}
+
 template <typename T>
 __device__ __forceinline__ void chi_apply(const PassArgs<T>& a, const cx<T>* chi_b, int64_t v, const cx<T>* x,
                                           cx<T>* y, const Coef<T>& cf) {
-    const cx<T> cc{cf.c_r, cf.c_i};
-    if (a.chi_kind == 1) {
-        const cx<T> x0 = chi_b[v];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = scal(x0 * x[c], cf.a) + cc * x[c];
-    } else if (a.chi_kind == 2) {
-        const int64_t V = a.comp_stride;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            cx<T> acc = chi_b[(r * 3 + 0) * V + v] * x[0];
-            acc = acc + chi_b[(r * 3 + 1) * V + v] * x[1];
-            acc = acc + chi_b[(r * 3 + 2) * V + v] * x[2];
-            y[r] = scal(acc, cf.a) + cc * x[r];
-        }
-    } else {
-        const int64_t V = a.comp_stride;
-        const cx<T> xx = chi_b[v], xy = chi_b[V + v], xz = chi_b[2 * V + v];
-        const cx<T> yy = chi_b[3 * V + v], yz = chi_b[4 * V + v], zz = chi_b[5 * V + v];
-        const cx<T> y0 = xx * x[0] + xy * x[1] + xz * x[2];
-        const cx<T> y1 = xy * x[0] + yy * x[1] + yz * x[2];
-        const cx<T> y2 = xz * x[0] + yz * x[1] + zz * x[2];
-        y[0] = scal(y0, cf.a) + cc * x[0];
-        y[1] = scal(y1, cf.a) + cc * x[1];
-        y[2] = scal(y2, cf.a) + cc * x[2];
+    ChiVox<T> q;
+    chi_load<T>(a, chi_b, v, q);
+    chi_mulv<T>(a, q, x, y, cf);
+}
+
+// Per-tile FP64 partial sums of |d|^2 and |Eout|^2 -> partials[2 blk + {0, 1}]; reduce_partials_kernel adds them
+// in tile order after the pass (deterministic). A last-block-done reduction inside the pass would need an
+// agent-scope release per block, which writes back the whole L2 of the XCD on this hardware.
+template <typename T>
+__device__ __forceinline__ void write_partials(double dd, double ee, const PassArgs<T>& a, int64_t blk, double* red) {
+    dd = block_sum(dd, red);
+    ee = block_sum(ee, red);
+    if (threadIdx.x == 0) {
+        a.partials[2 * blk + 0] = dd;
+        a.partials[2 * blk + 1] = ee;
     }
 }
 
@@ -159,13 +199,25 @@ enum PassKind : int {
     kGreen = 3,   // load, forward FFT, Green's operator, inverse FFT, store
     kInvEpi = 4,  // load, inverse FFT, Born epilogue (update + norms)
     kAll = 5,     // Born prologue, forward, Green, inverse, Born epilogue (single-axis grids)
+    kEpiPro = 6,  // load, inverse FFT, Born epilogue, then the next iteration's prologue and forward FFT, store
+    kGreenEpiPro = 7,  // single-axis grids: load (k-space), Green, inverse, epilogue, prologue, forward, store
+};
+
+template <int KIND>
+struct PassFlags {
+    static constexpr bool LOAD_PRO = (KIND == kProFwd || KIND == kAll);
+    static constexpr bool FWD_FIRST = (KIND == kFwd || KIND == kProFwd || KIND == kGreen || KIND == kAll);
+    static constexpr bool GREEN = (KIND == kGreen || KIND == kAll || KIND == kGreenEpiPro);
+    static constexpr bool INV = (KIND == kInv || KIND == kGreen || KIND == kInvEpi || KIND == kAll ||
+                                 KIND == kEpiPro || KIND == kGreenEpiPro);
+    static constexpr bool EPI = (KIND == kInvEpi || KIND == kAll || KIND == kEpiPro || KIND == kGreenEpiPro);
+    static constexpr bool PRO_AFTER = (KIND == kEpiPro || KIND == kGreenEpiPro);
+    static constexpr bool STORE = !(KIND == kInvEpi || KIND == kAll);
 };
 
 template <typename T, int E, int KIND>
 __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
-    constexpr bool PRO = (KIND == kProFwd || KIND == kAll);
-    constexpr bool GREEN = (KIND == kGreen || KIND == kAll);
-    constexpr bool EPI = (KIND == kInvEpi || KIND == kAll);
+    using F = PassFlags<KIND>;
     extern __shared__ unsigned char smem_raw[];
     __shared__ double red[16];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
@@ -184,7 +236,12 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     }
     const int64_t b = fdiv64(blk, a.ftpp);
     const int64_t t = blk - b * a.tiles_per_problem;
-    const Coef<T> cf = (a.coef != nullptr) ? a.coef[b] : Coef<T>{T(0), T(1), T(1), T(0), T(0), T(0)};
+    Coef<T> cf;
+    if (a.coef != nullptr) {
+        cf = a.coef[b];
+    } else {
+        cf.alpha_r = T(0); cf.alpha_i = T(1); cf.a = T(1); cf.c_r = T(0); cf.c_i = T(0); cf.pad_ = T(0);
+    }
 
     TileGeom g;
     g.N = N;
@@ -203,7 +260,7 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     const int64_t pb = b * a.bstride;  // problem offset in src/dst (fused passes; 0 when folded)
 
     // ---------------- load ----------------
-    if constexpr (PRO) {
+    if constexpr (F::LOAD_PRO) {
         // rows tile with all C components; voxel (row, j)
         const cx<T>* Eb = a.E + b * a.E_bstride;
         const cx<T>* sb = a.s + b * a.s_bstride;
@@ -257,8 +314,8 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     __syncthreads();
 
     // ---------------- FFT (and Green) ----------------
-    if constexpr (GREEN) {
-        tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
+    if constexpr (F::FWD_FIRST) tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
+    if constexpr (F::GREEN) {
         const int nvox = a.P * N;
         for (int idx = tid; idx < nvox; idx += nth) {
             int p, j;
@@ -278,16 +335,13 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
             for (int c = 0; c < 3; ++c) if (c < a.ncomp) tile[j * g.SJ + (c * a.P + p) * g.SM] = f[c];
         }
         __syncthreads();
-        tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
-    } else if constexpr (KIND == kFwd || KIND == kProFwd) {
-        tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
-    } else {
-        tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
     }
+    if constexpr (F::INV) tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
 
     // ---------------- store ----------------
-    if constexpr (EPI) {
-        // rows tile with all C components: d = chi_eff (W - E), E' = E + d
+    if constexpr (F::EPI) {
+        // rows tile with all C components: d = chi_eff (W - E), E' = E + d (then, for the fused kinds,
+        // the next iteration's x = chi_eff E' + s_eff goes back into the tile)
         const cx<T>* Eb = a.E + b * a.E_bstride;
         const cx<T>* chib = a.chi + b * a.chi_bstride;
         cx<T>* Eo = a.Eout + b * a.E_bstride;
@@ -307,22 +361,32 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
                 e[c] = Eb[c * V + v];
                 w[c] = tile[(c * a.P + p) * g.SM + j] - e[c];
             }
-            chi_apply<T>(a, chib, v, w, d, cf);
+            ChiVox<T> q;
+            chi_load<T>(a, chib, v, q);
+            chi_mulv<T>(a, q, w, d, cf);
+            cx<T> en[3] = {z, z, z};
 #pragma unroll
             for (int c = 0; c < 3; ++c) if (c < a.C) {
-                const cx<T> en = e[c] + d[c];
-                Eo[c * V + v] = en;
+                en[c] = e[c] + d[c];
+                Eo[c * V + v] = en[c];
                 dd += norm2_f64(d[c]);
-                ee += norm2_f64(en);
+                ee += norm2_f64(en[c]);
+            }
+            if constexpr (F::PRO_AFTER) {
+                const cx<T>* sb = a.s + b * a.s_bstride;
+                cx<T> x[3] = {z, z, z};
+                chi_mulv<T>(a, q, en, x, cf);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) if (c < a.C) tile[(c * a.P + p) * g.SM + j] = x[c] + scal(sb[c * V + v], cf.a);
             }
         }
-        dd = block_sum(dd, red);
-        ee = block_sum(ee, red);
-        if (tid == 0) {
-            a.partials[2 * blk + 0] = dd;
-            a.partials[2 * blk + 1] = ee;
+        write_partials(dd, ee, a, blk, red);
+        if constexpr (F::PRO_AFTER) {
+            __syncthreads();
+            tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
         }
-    } else {
+    }
+    if constexpr (F::STORE) {
       if (rows) {
         const int total = a.ncomp * a.P * N;
         for (int idx = tid; idx < total; idx += nth) {
@@ -348,7 +412,7 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
 }
 
 // Sum of per-tile partials -> out[2*b + {0,1}], fixed order (deterministic). One block per problem.
-__global__ void __launch_bounds__(256) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
+__global__ void __launch_bounds__(512) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
     __shared__ double red[16];
     const int64_t b = blockIdx.x;
     const double* pp = partials + 2 * b * tiles;

@@ -10,11 +10,81 @@
 
 namespace mmx {
 
-template <typename T, int N, int E, int KIND, int NC>
+// Element at a 32-bit byte offset from a wave-uniform base: the compiler keeps the base in SGPRs and issues
+// global loads/stores with a single 32-bit VGPR offset (no 64-bit address arithmetic per access).
+template <typename T> __device__ __forceinline__ const cx<T>& at(const cx<T>* base, uint32_t boff) {
+    return *reinterpret_cast<const cx<T>*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T> __device__ __forceinline__ cx<T>& at(cx<T>* base, uint32_t boff) {
+    return *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(base) + boff);
+}
+
+// chi of one voxel for a compile-time chi kind CK (1: scalar, 2: full 3x3 row-major, 3: packed symmetric).
+template <typename T, int CK>
+__device__ __forceinline__ void chi_load_ck(const cx<T>* chi_t, int64_t V, uint32_t boff, ChiVox<T>& q) {
+    q.m0 = at(chi_t, boff);
+    if constexpr (CK >= 2) {
+        q.m1 = at(chi_t + V, boff);
+        q.m2 = at(chi_t + 2 * V, boff);
+        q.m3 = at(chi_t + 3 * V, boff);
+        q.m4 = at(chi_t + 4 * V, boff);
+        q.m5 = at(chi_t + 5 * V, boff);
+    }
+    if constexpr (CK == 2) {
+        q.m6 = at(chi_t + 6 * V, boff);
+        q.m7 = at(chi_t + 7 * V, boff);
+        q.m8 = at(chi_t + 8 * V, boff);
+    }
+}
+
+// y = chi_eff x = a chi x + c x for NC components
+template <typename T, int CK, int NC>
+__device__ __forceinline__ void chi_mul_ck(const ChiVox<T>& q, const cx<T>* x, cx<T>* y, const Coef<T>& cf) {
+    const cx<T> cc{cf.c_r, cf.c_i};
+    if constexpr (CK == 1) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] = scal(q.m0 * x[c], cf.a) + cc * x[c];
+    } else {
+        cx<T> y0, y1, y2;
+        if constexpr (CK == 2) {
+            y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
+            y1 = q.m3 * x[0] + q.m4 * x[1] + q.m5 * x[2];
+            y2 = q.m6 * x[0] + q.m7 * x[1] + q.m8 * x[2];
+        } else {  // packed symmetric {xx, xy, xz, yy, yz, zz}
+            y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
+            y1 = q.m1 * x[0] + q.m3 * x[1] + q.m4 * x[2];
+            y2 = q.m2 * x[0] + q.m4 * x[1] + q.m5 * x[2];
+        }
+        y[0] = scal(y0, cf.a) + cc * x[0];
+        y[1] = scal(y1, cf.a) + cc * x[1];
+        y[2] = scal(y2, cf.a) + cc * x[2];
+    }
+}
+
+// Inputs of one voxel for the update (and the fused next prologue): E, chi and optionally s.
+template <typename T, int NC, bool WITH_S> struct EpiIn {
+    cx<T> e[NC];
+    ChiVox<T> q;
+    cx<T> s[WITH_S ? NC : 1];
+};
+
+template <typename T, int NC, int CK, bool WITH_S>
+__device__ __forceinline__ void epi_load(EpiIn<T, NC, WITH_S>& in, const cx<T>* Et, const cx<T>* chit,
+                                         const cx<T>* st, int64_t V, uint32_t bo) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) in.e[c] = at(Et + c * V, bo);
+    chi_load_ck<T, CK>(chit, V, bo, in.q);
+    if constexpr (WITH_S) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) in.s[c] = at(st + c * V, bo);
+    }
+}
+
+// The host launches this kernel only when (N * inner + P) * sizeof(cx<T>) < 2^32, so every offset within a tile
+// fits the 32-bit byte offsets used below. CK is the chi kind of the passes that apply chi (0 for the others).
+template <typename T, int N, int E, int KIND, int NC, int CK>
 __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
-    constexpr bool PRO = (KIND == kProFwd || KIND == kAll);
-    constexpr bool GREEN = (KIND == kGreen || KIND == kAll);
-    constexpr bool EPI = (KIND == kInvEpi || KIND == kAll);
+    using F = PassFlags<KIND>;
     constexpr int TP = N / E;
     constexpr int LTP = ilog2(TP);
     extern __shared__ unsigned char smem_raw[];
@@ -33,7 +103,12 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     }
     const int64_t b = fdiv64(blk, a.ftpp);
     const int64_t tl = blk - b * a.tiles_per_problem;
-    const Coef<T> cf = (a.coef != nullptr) ? a.coef[b] : Coef<T>{T(0), T(1), T(1), T(0), T(0), T(0)};
+    Coef<T> cf;
+    if (a.coef != nullptr) {
+        cf = a.coef[b];
+    } else {
+        cf.alpha_r = T(0); cf.alpha_i = T(1); cf.a = T(1); cf.c_r = T(0); cf.c_i = T(0); cf.pad_ = T(0);
+    }
 
     int p, t;
     cx<T>* xb;
@@ -45,9 +120,12 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     if (rows) r0 = tl * P;
     else { o = fdiv64(tl, a.fntile); ib = (tl - o * a.ntile_inner) * P; }
     const bool valid = rows ? (r0 + p < a.nouter) : (ib + p < a.inner);
-    // offset of point j of this thread's pencil within one component of one problem
-    const int64_t pbase = rows ? (r0 + p) * N : o * (int64_t)N * a.inner + ib + p;
-    const int64_t jstride = rows ? 1 : a.inner;
+    // element offset of the tile within one component of one problem (uniform), and the byte offset of this
+    // thread's register m from it: toff + m * mstep
+    constexpr uint32_t ES = sizeof(cx<T>);
+    const int64_t tb = rows ? r0 * N : o * (int64_t)N * a.inner + ib;
+    const uint32_t toff = rows ? (uint32_t)(p * N + t) * ES : ((uint32_t)p + (uint32_t)t * (uint32_t)a.inner) * ES;
+    const uint32_t mstep = rows ? (uint32_t)TP * ES : (uint32_t)TP * (uint32_t)a.inner * ES;
     const int64_t pb = b * a.bstride;
 
     for (int q = tid; q < N; q += blockDim.x) tw[q] = a.tw[q];
@@ -55,36 +133,46 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 
     cx<T> v[NC][E];
     const cx<T> zero{T(0), T(0)};
-    if constexpr (PRO) {
-        const cx<T>* Eb = a.E + b * a.E_bstride;
-        const cx<T>* sb = a.s + b * a.s_bstride;
-        const cx<T>* chib = a.chi + b * a.chi_bstride;
+    if constexpr (F::LOAD_PRO) {
         const int64_t V = a.comp_stride;
+        const cx<T>* Et = a.E + b * a.E_bstride + tb;
+        const cx<T>* st = a.s + b * a.s_bstride + tb;
+        const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             cx<T> x[3] = {zero, zero, zero}, y[3] = {zero, zero, zero};
             if (valid) {
-                const int64_t vox = pbase + (int64_t)(t + m * TP) * jstride;
+                const uint32_t bo = toff + m * mstep;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) x[c] = Eb[c * V + vox];
-                chi_apply<T>(a, chib, vox, x, y, cf);
+                for (int c = 0; c < NC; ++c) x[c] = at(Et + c * V, bo);
+                ChiVox<T> qv;
+                chi_load_ck<T, CK>(chit, V, bo, qv);
+                chi_mul_ck<T, CK, NC>(qv, x, y, cf);
 #pragma unroll
-                for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(sb[c * V + vox], cf.a);
+                for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(at(st + c * V, bo), cf.a);
             }
 #pragma unroll
             for (int c = 0; c < NC; ++c) v[c][m] = y[c];
+            if (NC == 3 && (m & 1)) asm volatile("" ::: "memory");  // bound the loads in flight per thread
         }
     } else {
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int64_t off = pbase + (int64_t)(t + m * TP) * jstride;
+        for (int m = 0; m < E; ++m)
 #pragma unroll
-            for (int c = 0; c < NC; ++c) v[c][m] = valid ? a.src[pb + c * a.comp_stride + off] : zero;
+            for (int c = 0; c < NC; ++c) v[c][m] = zero;
+        if (valid) {
+            const cx<T>* srct = a.src + pb + tb;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const uint32_t bo = toff + m * mstep;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) v[c][m] = at(srct + c * a.comp_stride, bo);
+            }
         }
     }
 
-    if constexpr (GREEN) {
-        reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    if constexpr (F::FWD_FIRST) reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    if constexpr (F::GREEN) {
         // k components off the pass axis are fixed for this thread's pencil
         int64_t col = rows ? 0 : ib + p;
         int64_t row = rows ? r0 + p : o;
@@ -111,52 +199,59 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) v[c][m] = f[c];
         }
-        reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
-    } else if constexpr (KIND == kFwd || KIND == kProFwd) {
-        reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
-    } else {
-        reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
     }
+    if constexpr (F::INV) reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
 
-    if constexpr (EPI) {
-        const cx<T>* Eb = a.E + b * a.E_bstride;
-        const cx<T>* chib = a.chi + b * a.chi_bstride;
-        cx<T>* Eo = a.Eout + b * a.E_bstride;
+    if constexpr (F::EPI) {
         const int64_t V = a.comp_stride;
+        const cx<T>* Et = a.E + b * a.E_bstride + tb;
+        const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
+        const cx<T>* st = a.s + b * a.s_bstride + tb;
+        cx<T>* Eot = a.Eout + b * a.E_bstride + tb;
         double dd = 0.0, ee = 0.0;
         if (valid) {
+            // Loads and stores retire in issue order on this hardware, so every input of voxel m + 1 (E, chi and,
+            // for the fused next prologue, s) is requested before voxel m's result is stored.
+            EpiIn<T, NC, F::PRO_AFTER> cur, nxt;
+            epi_load<T, NC, CK, F::PRO_AFTER>(cur, Et, chit, st, V, toff);
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                const int64_t vox = pbase + (int64_t)(t + m * TP) * jstride;
-                cx<T> e[3] = {zero, zero, zero}, w[3] = {zero, zero, zero}, d[3] = {zero, zero, zero};
+                if (m + 1 < E) epi_load<T, NC, CK, F::PRO_AFTER>(nxt, Et, chit, st, V, toff + (m + 1) * mstep);
+                const uint32_t bo = toff + m * mstep;
+                cx<T> w[3] = {zero, zero, zero}, d[3] = {zero, zero, zero}, en[3] = {zero, zero, zero};
+#pragma unroll
+                for (int c = 0; c < NC; ++c) w[c] = v[c][m] - cur.e[c];
+                chi_mul_ck<T, CK, NC>(cur.q, w, d, cf);
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
-                    e[c] = Eb[c * V + vox];
-                    w[c] = v[c][m] - e[c];
-                }
-                chi_apply<T>(a, chib, vox, w, d, cf);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    const cx<T> en = e[c] + d[c];
-                    Eo[c * V + vox] = en;
+                    en[c] = cur.e[c] + d[c];
+                    at(Eot + c * V, bo) = en[c];
                     dd += norm2_f64(d[c]);
-                    ee += norm2_f64(en);
+                    ee += norm2_f64(en[c]);
                 }
+                if constexpr (F::PRO_AFTER) {
+                    // the next iteration's x = chi_eff E' + s_eff, with chi already in registers
+                    cx<T> x[3] = {zero, zero, zero};
+                    chi_mul_ck<T, CK, NC>(cur.q, en, x, cf);
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) v[c][m] = x[c] + scal(cur.s[c], cf.a);
+                }
+                if (m + 1 < E) cur = nxt;
             }
         }
-        dd = block_sum(dd, red);
-        ee = block_sum(ee, red);
-        if (tid == 0) {
-            a.partials[2 * blk + 0] = dd;
-            a.partials[2 * blk + 1] = ee;
-        }
-    } else if (valid) {
+        write_partials(dd, ee, a, blk, red);
+        if constexpr (F::PRO_AFTER) reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    }
+    if constexpr (F::STORE) {
+      if (valid) {
+        cx<T>* dstt = a.dst + pb + tb;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            const int64_t off = pbase + (int64_t)(t + m * TP) * jstride;
+            const uint32_t bo = toff + m * mstep;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) a.dst[pb + c * a.comp_stride + off] = scal(v[c][m], a.scale);
+            for (int c = 0; c < NC; ++c) at(dstt + c * a.comp_stride, bo) = scal(v[c][m], a.scale);
         }
+      }
     }
 }
 

@@ -107,6 +107,8 @@ class Solution:
         BE = self.__BE
         self.__field_array = BE.allocate_array(fill_value=0)
         self.__field_next = BE.allocate_array(fill_value=0)
+        self.__work = BE.allocate_array(fill_value=0)  # first pass of the next iteration (FFT along axis -1)
+        self.__work_valid = False
         self.__norms = torch.zeros(2, dtype=torch.float64, device=BE.device)
         self.E = initial_field
 
@@ -214,6 +216,7 @@ class Solution:
         self.source_distribution = source_distribution
 
     def __upload_coef(self):
+        self.__work_valid = False
         a = self.__alpha
         vals = [a.real, a.imag, self.__coef_a, self.__coef_c.real, self.__coef_c.imag, 0.0]
         self.__coef.copy_(torch.tensor(vals, dtype=torch.float64))
@@ -339,6 +342,7 @@ class Solution:
         self.__source_normalized = (S * (1.0j / self.__alpha_base.imag / self.__beta)).to(
             BE.hardware_dtype).contiguous()
         self.__previous_update_norm = math.inf
+        self.__work_valid = False
 
     @property
     def j(self) -> np.ndarray:
@@ -395,6 +399,7 @@ class Solution:
         BE.assign(t, self.__field_array)
         self.__previous_update_norm = math.inf
         self.__field_norm = None
+        self.__work_valid = False
 
     @property
     def field_array(self) -> torch.Tensor:
@@ -463,8 +468,13 @@ class Solution:
         self.__iteration += 1
         self.__relative_residue = None
         BE = self.__BE
-        BE.born_step(self.__field_array, self.__field_next, self.__source_normalized, self.__chi, self.__chi_kind,
-                     self.__coef, self.__norms)
+        if not self.__work_valid:
+            BE.born_prologue(self.__field_array, self.__work, self.__source_normalized, self.__chi, self.__chi_kind,
+                             self.__coef)
+        # completes this iteration and, speculating that it is accepted, computes the next one's first pass
+        BE.born_iterate(self.__field_array, self.__field_next, self.__work, self.__source_normalized, self.__chi,
+                        self.__chi_kind, self.__coef, self.__norms)
+        self.__work_valid = True
         d2, e2 = self.__norms.tolist()  # the one host sync of the iteration
         current = math.sqrt(d2)
         with np.errstate(divide='ignore', invalid='ignore'):
@@ -477,7 +487,7 @@ class Solution:
         log.warning(f'The field update is scaled by {rel:0.3f} >= 1 in iteration {self.__iteration}.')
         alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)
         log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
-        self.__update_operators(alpha_new)
+        self.__update_operators(alpha_new)  # also marks the speculative first pass in the work buffer stale
         return False
 
     def __iter__(self):

@@ -91,6 +91,8 @@ BORN_CASES = [
     ((32, 32, 32), 1, 1), ((24, 32, 40), 3, 1), ((16, 20, 24), 3, 2), ((5, 10, 20), 3, 2),
     ((14, 21, 8), 3, 1), ((8, 1, 12), 3, 2), ((64, 64, 64), 3, 2),
     ((16, 20, 24), 3, 3), ((64, 48), 3, 3), ((512,), 3, 3),
+    # power-of-two axes (register-resident kernels) with every chi kind
+    ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
 ]
 
 
@@ -179,3 +181,41 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
     plan.norm2(Ft, x)
     F32 = F.astype(np.complex64).astype(np.complex128)  # the values the device holds, summed in float64
     assert abs(float(x.item()) - np.sum(np.abs(F32) ** 2)) / np.sum(np.abs(F32) ** 2) < 1e-12
+
+
+@pytest.mark.parametrize('shape,C,kind,dtype', [((256, 256, 256), 3, 3, 'c64'), ((64, 64, 64), 3, 2, 'c64'),
+                                                ((96, 40, 24), 3, 1, 'c64'), ((128, 96), 1, 1, 'c64'),
+                                                ((1024,), 3, 2, 'c64'), ((512,), 3, 3, 'c64'),
+                                                ((64, 128, 64), 3, 1, 'c64'), ((128, 64, 256), 3, 3, 'c64'),
+                                                ((64, 64, 128), 1, 1, 'c64'), ((256, 64), 3, 2, 'c64'),
+                                                ((16, 20, 24), 3, 3, 'c128'), ((48, 64), 1, 1, 'c128')],
+                         ids=lambda v: str(v))
+def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
+    """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
+    import torch
+    print(f'prologue/iterate {shape} C={C} kind={kind} {dtype}', flush=True)
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, seed=5)
+    step = [0.41] * len(shape)
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    rdt = torch.float32 if dtype == 'c64' else torch.float64
+    tol = 2e-6 if dtype == 'c64' else 1e-12
+    plan = _plan(shape, step, tdt, device)
+    Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
+    st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind), dtype=tdt, device=device).contiguous()
+    coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
+    n1 = torch.zeros(2, dtype=torch.float64, device=device)
+    n2 = torch.zeros(2, dtype=torch.float64, device=device)
+    E1 = torch.empty_like(Et)
+    plan.born_step(Et, E1, st, ct, C, kind, coef, 1, 0, 0, n1)
+    W = torch.empty_like(Et)
+    E2 = torch.empty_like(Et)
+    plan.born_prologue(Et, W, st, ct, C, kind, coef)
+    plan.born_iterate(Et, E2, W, st, ct, C, kind, coef, n2)
+    W_next = torch.empty_like(Et)
+    plan.born_prologue(E2, W_next, st, ct, C, kind, coef)
+    torch.cuda.synchronize()
+    d1 = (E1 - Et)
+    assert float(torch.linalg.norm(E2 - E1) / torch.linalg.norm(d1)) < tol
+    assert float(((n2 - n1).abs() / n1.abs()).max()) < 10 * tol
+    assert float(torch.linalg.norm(W - W_next) / torch.linalg.norm(W_next)) < tol
