@@ -49,7 +49,9 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *   MMX_PARAM_P_ROWS     pencils per tile in passes along the contiguous axis
  *   MMX_PARAM_P_STRIDED  pencils (contiguous columns) per tile in single-component strided passes
  *   MMX_PARAM_P_GREEN    pencils per tile in the fused Green pass (all components in one tile)
- *   MMX_PARAM_XCD_REMAP  1: remap workgroups so that neighbouring tiles run on one XCD; 0: identity
+ *   MMX_PARAM_XCD_REMAP  bit mask over pass kinds (bits as for MMX_PARAM_BIG_ELEMS, plus 6 inverse+update+next
+ *                        prologue, 7 single-axis with next prologue): those passes remap workgroups so that
+ *                        neighbouring tiles run on one XCD (default: Green and update passes)
  *   MMX_PARAM_LDS_BUDGET preferred LDS bytes per workgroup
  *   MMX_PARAM_BIG_ELEMS  bit mask over pass kinds (bit 0 forward, 1 inverse, 2 prologue+forward, 3 Green,
  *                        4 inverse+update, 5 single-axis): those passes give each thread twice the elements

@@ -351,7 +351,7 @@ int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, 
     if (!fused) a.ncomp = 1;
     int rc = reg_tile_config(ax, a.ncomp, fused, nbatch, preq, kind, t);
     if (rc) return rc;
-    a.xcd_remap = p->xcd_remap;
+    a.xcd_remap = (p->xcd_remap >> kind) & 1;
     if (!fused) a.bstride = 0;
     a.nouter = t.nouter;
     a.P = t.P;
@@ -392,7 +392,7 @@ int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t 
     int rc = tile_config<T>(ax, a.ncomp, fused, nbatch, p->lds_max, t, preq, p->lds_budget,
                             (p->big_elems >> kind) & 1);
     if (rc) return rc;
-    a.xcd_remap = p->xcd_remap;
+    a.xcd_remap = (p->xcd_remap >> kind) & 1;
     if (!fused) {
         a.ncomp = 1;
         a.bstride = 0;
@@ -751,7 +751,9 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
         p->lds_max = (size_t)v;
     }
     p->g_axis = 1;  // Green pass on the middle axis of 3-D grids (clamped to the outermost axis otherwise)
-    p->xcd_remap = 0;
+    // measured at 256^3: the strided Green pass and the update passes gain from running neighbouring tiles on one
+    // XCD (their 64-byte row segments share L2 lines); the plain passes do not
+    p->xcd_remap = (1 << kGreen) | (1 << kInvEpi) | (1 << kEpiPro) | (1 << kGreenEpiPro);
     p->reg_fft = 1;
     p->lds_budget = kLdsBudget;
     // non-trivial axes, outermost first
@@ -814,7 +816,10 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
         case MMX_PARAM_P_ROWS: p->p_rows = (int)value; break;
         case MMX_PARAM_P_STRIDED: p->p_strided = (int)value; break;
         case MMX_PARAM_P_GREEN: p->p_green = (int)value; break;
-        case MMX_PARAM_XCD_REMAP: p->xcd_remap = value ? 1 : 0; break;
+        case MMX_PARAM_XCD_REMAP:
+            if (value < 0 || value > 255) return fail(MMX_EINVAL, "XCD remap mask must be in [0, 255]");
+            p->xcd_remap = (int)value;
+            break;
         case MMX_PARAM_LDS_BUDGET: p->lds_budget = value ? (size_t)value : kLdsBudget; break;
         case MMX_PARAM_BIG_ELEMS: p->big_elems = (int)value; break;
         case MMX_PARAM_REG_FFT: p->reg_fft = value ? 1 : 0; break;

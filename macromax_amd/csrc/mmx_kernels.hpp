@@ -80,30 +80,33 @@ struct PassArgs {
 // backend/__init__.py:606-649): g = -i alpha_i / (k^2 - alpha);
 // vectorial: F <- g (F - Pi_L F) + (i alpha_i / alpha) Pi_L F, Pi_L = k k^T / k^2, Pi_L(0) = I on the
 // first nd_data components and 0 on the others. The result is scaled by 1/N_total.
+// 1/x: the hardware reciprocal (1 ulp) in single precision, a correctly rounded division in double.
+template <typename T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T>
 __device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, bool dc, const PassArgs<T>& a,
                                             const Coef<T>& cf) {
-    const cx<T> alpha{cf.alpha_r, cf.alpha_i};
     const T kk = k0 * k0 + k1 * k1 + k2 * k2;
-    // g = -i alpha_i / (kk - alpha), times 1/N
-    const cx<T> den{kk - alpha.r, -alpha.i};
-    const cx<T> num{T(0), -cf.alpha_i * a.inv_ntotal};
-    const cx<T> g = cdiv(num, den);
+    // g = -i alpha_i / (kk - alpha) / N = alpha_i (alpha_i - i x) / (x^2 + alpha_i^2) / N with x = kk - alpha_r
+    const T x = kk - cf.alpha_r;
+    const T r = cf.alpha_i * a.inv_ntotal * recip(x * x + cf.alpha_i * cf.alpha_i);
+    const cx<T> g{cf.alpha_i * r, -x * r};
     if (!a.vectorial || C == 1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) if (c < C) f[c] = g * f[c];
         return;
     }
-    // i alpha_i / alpha, times 1/N
-    const cx<T> gl = cdiv(cx<T>{T(0), cf.alpha_i * a.inv_ntotal}, alpha);
+    // i alpha_i / alpha / N = alpha_i (alpha_i + i alpha_r) / |alpha|^2 / N
+    const T rl = cf.alpha_i * a.inv_ntotal / (cf.alpha_r * cf.alpha_r + cf.alpha_i * cf.alpha_i);
+    const cx<T> gl{cf.alpha_i * rl, cf.alpha_r * rl};
     cx<T> pl[3];
     if (dc) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) pl[c] = (c < a.nd_data) ? f[c] : cx<T>{T(0), T(0)};
     } else {
-        const T inv = T(1) / kk;
         cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
-        proj = scal(proj, inv);
+        proj = scal(proj, recip(kk));
         pl[0] = scal(proj, k0);
         pl[1] = scal(proj, k1);
         pl[2] = scal(proj, k2);
