@@ -7,7 +7,8 @@ LinearBound on z (periodic x, y); complex64 fields, FP64 norms.
 
 A "step" is one Born iteration (Solution.step: the fused HIP kernel sequence plus the accept/reject decision).
 `value` is voxel-iterations/s with all inputs resident in HBM. The JSON line also carries the HBM roofline of the
-iteration (algorithmic bytes per voxel-iteration, 480 B for this workload) and a CPU baseline: the NumPy
+iteration (algorithmic bytes per voxel-iteration, 480 B for this workload; `traffic` is the HBM bytes per iteration
+measured with rocprofv3 counters, from profiles/born_iterate_256cubed_aniso.md) and a CPU baseline: the NumPy
 complex128 oracle iteration on this host (single-threaded numpy.fft), timed for one iteration of the same grid.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--grid N]
@@ -92,6 +93,17 @@ def cpu_baseline(sol, n):
             'os_cpu_count': os.cpu_count(), 'affinity_cores': len(os.sched_getaffinity(0))}
 
 
+def measured_traffic(n):
+    """HBM bytes per voxel-iteration from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
+    path = os.path.join(ROOT, 'profiles', 'born_iterate_256cubed_aniso.md')
+    if n != 256 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith('traffic_bytes_per_voxel_iteration:'):
+            return float(line.split(':')[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -153,10 +165,16 @@ def main():
     b_alg = 480.0  # bytes per voxel-iteration: 4*d*C*e + 2*B_chi + B_s + C*e with d=3, C=3, e=8, B_chi=72
     gpu_ms_per_iter = gpu_ms / args.steps
     achieved = b_alg * V / (gpu_ms_per_iter * 1e-3) / 1e9
+    traffic = measured_traffic(n)
     roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-                'kernel': 'one Born iteration = 5 pencil-pass kernels + norm reduction (mmx_born_step)',
+                'frac': round(achieved / HBM_PEAK_GBS, 4),
+                'traffic': None if traffic is None else round(traffic * V),
+                'kernel': 'mmx_born_iterate: one Born iteration = 4 pencil-pass kernels + FP64 norm reduction, '
+                          'timed with HIP events on its launch stream',
                 'bytes_per_voxel_iter_alg': b_alg, 'gpu_ms_per_iter_events': round(gpu_ms_per_iter, 4)}
+    if traffic is not None:
+        roofline['traffic_bytes_per_voxel_iter'] = traffic
+        roofline['actual_GBps'] = round(traffic * V / (gpu_ms_per_iter * 1e-3) / 1e9, 1)
 
     out = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'voxel-iter/s', 'n_gpus': 1, 'steps': args.steps,

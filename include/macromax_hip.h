@@ -120,6 +120,51 @@ int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const 
                      int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
                      double* norms, void* stream);
 
+/*
+ * Batched solves: nbatch problems of one grid shape (many right-hand sides of one medium when chi_bstride = 0, or
+ * many media) iterate together with the accept/reject decision and the stop test taken on the device, so that a
+ * run of calls can be captured in a graph and replayed without host synchronisation. No single reference
+ * function corresponds: it replaces the host loop of Solution.__iter__ / Solution.solve (solver.py:1064-1123)
+ * that ScatteringMatrix runs once per right-hand side (matrix.py:789-819), with the same rules per problem.
+ *   EA, EB: [nbatch, C, 1, *grid] field buffers; problem b's current field is in EB when bit 0 of state[b] is set.
+ *   W:      [nbatch, C, 1, *grid] work buffer (the first pass of the next iteration).
+ *   coef:   nbatch records as for mmx_born_step; the device rewrites a record when it raises alpha_i.
+ *   state:  nbatch words: bit 0 (MMX_STATE_PARITY) field in EB, bit 1 (MMX_STATE_DONE) finished, bit 2
+ *           (MMX_STATE_STALE) W must be recomputed (set it for every problem before the first call).
+ *   track:  nbatch records of MMX_TRACK_LEN doubles, indexed by MMX_TRACK_*; set PREV = +inf, ITER = 0, ALPHA_*,
+ *           A = 1, C_* = 0 (or the coef values) before the first call.
+ *   norms:  2 * nbatch doubles of scratch.
+ * One call runs one iteration of every unfinished problem: the first pass where W is stale, the remaining passes,
+ * then per problem: accept if ||d|| < ||d_prev|| (swap by the parity bit), else alpha_i <- 1.5 alpha_i
+ * (solver.py:1083-1096); residue = ||d_prev|| / ||E||; done when !(residue > tol && iterations < max_iter).
+ * The first call with a given plan and nbatch may allocate scratch; later calls allocate nothing and issue only
+ * kernel launches on `stream`.
+ */
+#define MMX_STATE_PARITY 1u
+#define MMX_STATE_DONE 2u
+#define MMX_STATE_STALE 4u
+#define MMX_TRACK_PREV 0
+#define MMX_TRACK_FIELD_NORM 1
+#define MMX_TRACK_RESIDUE 2
+#define MMX_TRACK_ITER 3
+#define MMX_TRACK_ALPHA_R 4
+#define MMX_TRACK_ALPHA_I 5
+#define MMX_TRACK_A 6
+#define MMX_TRACK_C_R 7
+#define MMX_TRACK_C_I 8
+#define MMX_TRACK_DONE_CLOCK 9
+#define MMX_TRACK_REJECTED 10
+#define MMX_TRACK_LEN 12
+int mmx_batch_step(mmx_plan_t plan, void* EA, void* EB, void* W, const void* s, const void* chi, int ncomp,
+                   int chi_kind, void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, uint32_t* state,
+                   double* track, double* norms, double tol, int64_t max_iter, void* stream);
+
+/* Device wall clock (constant rate, see mmx_wall_clock_rate_khz) written to out[0] by a one-thread kernel on
+ * `stream`; the batch controller stamps MMX_TRACK_DONE_CLOCK with the same clock. No reference counterpart
+ * (the reference times solves on the host, examples/benchmark.py:66-84). */
+int mmx_wall_clock(double* out, void* stream);
+int mmx_wall_clock_rate_khz(int64_t* khz);
+
 /* out[0] = sum |x|^2 over n complex elements, FP64 accumulation (backend/__init__.py:872-874, torch.py:316-317). */
 int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stream);
 

@@ -18,8 +18,14 @@ MMX_C128 = 1
 
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
             'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue', 'mmx_born_iterate',
-            'mmx_norm2', 'mmx_chi_mul',
+            'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
+
+# batched-solve state bits and per-problem track record (include/macromax_hip.h, MMX_STATE_* / MMX_TRACK_*)
+STATE_PARITY, STATE_DONE, STATE_STALE = 1, 2, 4
+TRACK = {'prev': 0, 'field_norm': 1, 'residue': 2, 'iter': 3, 'alpha_r': 4, 'alpha_i': 5, 'a': 6, 'c_r': 7,
+         'c_i': 8, 'done_clock': 9, 'rejected': 10}
+TRACK_LEN = 12
 
 PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6,
           'big_elems': 7, 'reg_fft': 8}
@@ -55,6 +61,13 @@ def _declare(L):
     L.mmx_born_prologue.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p]
     L.mmx_born_iterate.restype = c_int
     L.mmx_born_iterate.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
+    L.mmx_batch_step.restype = c_int
+    L.mmx_batch_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p,
+                                 ctypes.c_double, c_i64, c_p]
+    L.mmx_wall_clock.restype = c_int
+    L.mmx_wall_clock.argtypes = [c_p, c_p]
+    L.mmx_wall_clock_rate_khz.restype = c_int
+    L.mmx_wall_clock_rate_khz.argtypes = [ctypes.POINTER(c_i64)]
     L.mmx_norm2.restype = c_int
     L.mmx_norm2.argtypes = [c_p, c_p, c_i64, c_p, c_p]
     L.mmx_chi_mul.restype = c_int
@@ -74,6 +87,13 @@ def lib():
                     raise ImportError(f'{LIB_PATH} not found: build it with `python -m macromax_amd.build`.')
                 _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def wall_clock_rate_khz() -> int:
+    """Rate of the device wall clock that mmx_wall_clock and the batch controller read."""
+    v = ctypes.c_int64()
+    check(lib().mmx_wall_clock_rate_khz(ctypes.byref(v)), 'mmx_wall_clock_rate_khz')
+    return int(v.value)
 
 
 def check(rc: int, what: str = ''):

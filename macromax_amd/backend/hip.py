@@ -179,6 +179,23 @@ class Plan:
                                                  norms.data_ptr(), _stream_ptr(self.device)), 'mmx_born_iterate')
         return norms
 
+    def batch_step(self, EA, EB, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, state, track,
+                   norms, tol, max_iter):
+        """One device-decided iteration of every unfinished problem of a batch (``mmx_batch_step``)."""
+        self._born_checks(EA, ((EB, 'EB'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
+                          chi_bstride)
+        if state.dtype != torch.int32 or state.numel() < nbatch or state.device != self.device:
+            raise ValueError('state must be an int32 device tensor of nbatch elements')
+        if track.dtype != torch.float64 or track.numel() < nbatch * _native.TRACK_LEN or track.device != self.device:
+            raise ValueError('track must be a float64 device tensor of nbatch x TRACK_LEN elements')
+        if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
+            raise ValueError('norms must be a float64 device tensor of 2*nbatch elements')
+        _native.check(self._lib.mmx_batch_step(self.handle, EA.data_ptr(), EB.data_ptr(), W.data_ptr(),
+                                               s.data_ptr(), chi.data_ptr(), int(ncomp), int(chi_kind),
+                                               coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
+                                               state.data_ptr(), track.data_ptr(), norms.data_ptr(), float(tol),
+                                               int(max_iter), _stream_ptr(self.device)), 'mmx_batch_step')
+
     def norm2(self, x, out):
         self._check(x, 'norm2 input')
         _native.check(self._lib.mmx_norm2(self.handle, x.data_ptr(), x.numel(), out.data_ptr(),

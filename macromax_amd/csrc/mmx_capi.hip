@@ -503,12 +503,16 @@ PassArgs<T> born_base(const mmx_plan_s* p, const void* E, void* Eout, const void
 // W = forward FFT along the innermost axis of chi_eff E + s_eff (the first pass of an iteration).
 template <typename T>
 int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const void* chi, int C, int chi_kind,
-                  const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, hipStream_t stream) {
+                  const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, hipStream_t stream,
+                  const unsigned* state = nullptr, const void* E_alt = nullptr) {
     int rc = born_check<T>(p, C, chi_kind);
     if (rc) return rc;
     if (nbatch <= 0) return MMX_OK;
     const AxisPlan& inner = p->ax[p->naxes - 1];
-    PassArgs<T> a = born_base<T>(p, E, nullptr, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, nullptr);
+    PassArgs<T> a = born_base<T>(p, E, const_cast<void*>(E_alt), s, chi, C, chi_kind, coef, s_bstride, chi_bstride,
+                                 nullptr);
+    a.state = state;
+    a.only_stale = state != nullptr;
     fill_geometry<T>(p, inner, a);
     a.ncomp = C;
     a.bstride = (int64_t)C * p->V;
@@ -522,7 +526,7 @@ int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const vo
 template <typename T>
 int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void* s, const void* chi, int C,
                  int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
-                 double* norms, hipStream_t stream) {
+                 double* norms, hipStream_t stream, const unsigned* state = nullptr) {
     int rc = born_check<T>(p, C, chi_kind);
     if (rc) return rc;
     if (nbatch <= 0) return MMX_OK;
@@ -537,6 +541,7 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
     if (rc) return rc;
     double* partials = reinterpret_cast<double*>(p->scratch);
     PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
+    base.state = state;
     cx<T>* W = reinterpret_cast<cx<T>*>(Wv);
     if (d >= 2) {
         int g = p->g_axis;
@@ -548,6 +553,7 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
         for (int q = 0; q < nothers; ++q) {
             PassArgs<T> a = base;
             fill_geometry<T>(p, p->ax[others[q]], a);
+            a.frpp = make_fdiv64((int64_t)C * p->ax[others[q]].nouter);
             a.src = W;
             a.dst = W;
             rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kFwd, stream);
@@ -566,6 +572,7 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
         for (int q = nothers - 1; q >= 0; --q) {
             PassArgs<T> a = base;
             fill_geometry<T>(p, p->ax[others[q]], a);
+            a.frpp = make_fdiv64((int64_t)C * p->ax[others[q]].nouter);
             a.src = W;
             a.dst = W;
             rc = launch_pass<T>(p, p->ax[others[q]], a, nbatch * C, false, kInv, stream);
@@ -675,6 +682,25 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
         }
     }
     return reduce_norms(partials, tiles_last, nbatch, norms, stream);
+}
+
+// One Born iteration of every unfinished problem of a batch, then the accept/reject decision and the stop test on
+// the device. EA and EB are the two field buffers (problem b's current field is in EB when its parity bit is set).
+template <typename T>
+int batch_step_impl(mmx_plan_s* p, void* EA, void* EB, void* W, const void* s, const void* chi, int C, int chi_kind,
+                    void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, unsigned* state,
+                    double* track, double* norms, double tol, int64_t max_iter, hipStream_t stream) {
+    if (nbatch <= 0) return MMX_OK;
+    int rc = prologue_impl<T>(p, EA, W, s, chi, C, chi_kind, coef, nbatch, s_bstride, chi_bstride, stream, state, EB);
+    if (rc) return rc;
+    rc = iterate_impl<T>(p, EA, EB, W, s, chi, C, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, stream,
+                         state);
+    if (rc) return rc;
+    const int nblk = (int)((nbatch + 255) / 256 < 1024 ? (nbatch + 255) / 256 : 1024);
+    hipLaunchKernelGGL((batch_control_kernel<T>), dim3(nblk), dim3(256), 0, stream, nbatch, norms, state,
+                       reinterpret_cast<Coef<T>*>(coef), track, tol, double(max_iter));
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
 }
 
 template <typename T>
@@ -887,6 +913,41 @@ int mmx_born_iterate(mmx_plan_t p, const void* E, void* Eout, void* W, const voi
     return p->dtype == MMX_C64
                ? iterate_impl<float>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
                : iterate_impl<double>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
+}
+
+int mmx_batch_step(mmx_plan_t p, void* EA, void* EB, void* W, const void* s, const void* chi, int ncomp,
+                   int chi_kind, void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, uint32_t* state,
+                   double* track, double* norms, double tol, int64_t max_iter, void* stream) {
+    g_last_error.clear();
+    if (!p || !EA || !EB || !W || !s || !chi || !coef || !state || !track || !norms)
+        return fail(MMX_EINVAL, "null argument");
+    if (EA == EB || EA == W || EB == W) return fail(MMX_EINVAL, "EA, EB and W must be different buffers");
+    if (max_iter < 0) return fail(MMX_EINVAL, "max_iter must be >= 0");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    unsigned* stt = reinterpret_cast<unsigned*>(state);
+    return p->dtype == MMX_C64
+               ? batch_step_impl<float>(p, EA, EB, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride,
+                                        stt, track, norms, tol, max_iter, st)
+               : batch_step_impl<double>(p, EA, EB, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride,
+                                         stt, track, norms, tol, max_iter, st);
+}
+
+int mmx_wall_clock(double* out, void* stream) {
+    g_last_error.clear();
+    if (!out) return fail(MMX_EINVAL, "null argument");
+    hipLaunchKernelGGL(wall_clock_kernel, dim3(1), dim3(1), 0, reinterpret_cast<hipStream_t>(stream), out);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+int mmx_wall_clock_rate_khz(int64_t* khz) {
+    g_last_error.clear();
+    if (!khz) return fail(MMX_EINVAL, "null argument");
+    int dev = 0, rate = 0;
+    MMX_HIP_CHECK(hipGetDevice(&dev));
+    MMX_HIP_CHECK(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev));
+    *khz = rate;
+    return MMX_OK;
 }
 
 int mmx_norm2(mmx_plan_t p, const void* x, int64_t n, double* out, void* stream) {

@@ -28,6 +28,11 @@ struct Coef {
     T pad_;
 };
 
+// Per-problem flags of batched solves (one 32-bit word per problem, owned by the device-side controller).
+constexpr unsigned kStateParity = 1u;  // the current field lives in the second field buffer (Eout)
+constexpr unsigned kStateDone = 2u;    // finished (residue or iteration limit): no pass touches the problem again
+constexpr unsigned kStateStale = 4u;   // the work buffer does not hold the first pass of the current field
+
 template <typename T>
 struct PassArgs {
     // axis geometry (per component, per problem)
@@ -65,6 +70,11 @@ struct PassArgs {
     double* partials;         // 2 per (problem, tile)
     double* norms;            // per problem: {sum |d|^2, sum |Eout|^2}
     const Coef<T>* coef;      // one per problem
+    // batched solves (state == nullptr otherwise): per-problem flags; passes skip finished problems, the first and
+    // last passes read the current field from E or Eout by the parity bit and write the next one to the other
+    const unsigned* state;
+    int only_stale;           // first-pass launch: only the problems whose work buffer is stale
+    FDiv64 frpp;              // single-component passes: outer index o -> problem (o / (C * outer extent))
     // Green's function
     int64_t n0, n1, n2;       // grid shape padded with trailing 1s
     const T* kvec;            // k_0 at [0, n0), k_1 at [k1ofs, +n1), k_2 at [k2ofs, +n2)
@@ -254,18 +264,31 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     g.fM.d = g.M;
     g.fM.inv = 1.0f / float(g.M);
 
-    for (int q = tid; q < N; q += nth) tw[q] = a.tw[q];
 
     // tile origin: rows -> first row r0; strided -> (outer block o, first inner index i0)
     int64_t r0 = 0, o = 0, ib = 0;
     if (rows) r0 = t * a.P;
     else { o = fdiv64(t, a.fntile); ib = (t - o * a.ntile_inner) * a.P; }
     const int64_t pb = b * a.bstride;  // problem offset in src/dst (fused passes; 0 when folded)
+    const cx<T>* Ecur = a.E;
+    cx<T>* Enext = a.Eout;
+    if (a.state != nullptr) {
+        // single-component passes hold pencils of one problem per tile (strided tiles never span problems)
+        constexpr bool single = (KIND == kFwd || KIND == kInv);
+        const unsigned st = a.state[single ? fdiv64(rows ? r0 : o, a.frpp) : b];
+        if ((st & kStateDone) || (a.only_stale && !(st & kStateStale))) return;
+        if (st & kStateParity) {
+            Ecur = a.Eout;
+            Enext = const_cast<cx<T>*>(a.E);
+        }
+    }
+
+    for (int q = tid; q < N; q += nth) tw[q] = a.tw[q];
 
     // ---------------- load ----------------
     if constexpr (F::LOAD_PRO) {
         // rows tile with all C components; voxel (row, j)
-        const cx<T>* Eb = a.E + b * a.E_bstride;
+        const cx<T>* Eb = Ecur + b * a.E_bstride;
         const cx<T>* sb = a.s + b * a.s_bstride;
         const cx<T>* chib = a.chi + b * a.chi_bstride;
         const int64_t V = a.comp_stride;
@@ -345,9 +368,9 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     if constexpr (F::EPI) {
         // rows tile with all C components: d = chi_eff (W - E), E' = E + d (then, for the fused kinds,
         // the next iteration's x = chi_eff E' + s_eff goes back into the tile)
-        const cx<T>* Eb = a.E + b * a.E_bstride;
+        const cx<T>* Eb = Ecur + b * a.E_bstride;
         const cx<T>* chib = a.chi + b * a.chi_bstride;
-        cx<T>* Eo = a.Eout + b * a.E_bstride;
+        cx<T>* Eo = Enext + b * a.E_bstride;
         const int64_t V = a.comp_stride;
         double dd = 0.0, ee = 0.0;
         const int nvox = a.P * N;
@@ -431,6 +454,73 @@ __global__ void __launch_bounds__(512) reduce_partials_kernel(const double* part
         out[2 * b + 1] = s1;
     }
 }
+
+// Per-problem record of a batched solve (doubles; index constants are mirrored in include/macromax_hip.h).
+constexpr int kTrPrev = 0;       // ||d|| of the last accepted update (+inf before the first)
+constexpr int kTrFieldNorm = 1;  // ||E|| after the last accepted update
+constexpr int kTrResidue = 2;    // ||d_prev|| / ||E||, 0 while ||d_prev|| = 0
+constexpr int kTrIter = 3;       // iterations done, accepted or not
+constexpr int kTrAlphaR = 4;     // current alpha and the affine chi/s coefficients, in FP64 (coef holds them in T)
+constexpr int kTrAlphaI = 5;
+constexpr int kTrA = 6;
+constexpr int kTrCr = 7;
+constexpr int kTrCi = 8;
+constexpr int kTrDoneClock = 9;  // device wall clock when the problem finished
+constexpr int kTrRejected = 10;  // rejected updates
+constexpr int kTrLen = 12;
+
+// Accept / reject and stop test of every unfinished problem after one batched iteration; the same rules as
+// Solution.__iter__ and Solution.solve on the host (MacroMax solver.py:1078-1096, 1100-1123): accept when
+// ||d|| < ||d_prev|| (the first update always), else alpha_i <- 1.5 alpha_i with chi_eff = a chi + c I rescaled
+// as in solver.py:514-528 and the work buffer marked stale; stop when !(residue > tol && iterations < max_iter).
+template <typename T>
+__global__ void __launch_bounds__(256) batch_control_kernel(int64_t nbatch, const double* norms, unsigned* state,
+                                                            Coef<T>* coef, double* track, double tol,
+                                                            double max_iter) {
+    for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nbatch; b += (int64_t)gridDim.x * blockDim.x) {
+        unsigned st = state[b];
+        if (st & kStateDone) continue;
+        double* tr = track + b * kTrLen;
+        st &= ~kStateStale;  // this iteration's last pass computed the next first pass from the field it wrote
+        const double it = tr[kTrIter] + 1.0;
+        tr[kTrIter] = it;
+        const double d = sqrt(norms[2 * b]);
+        if (d < tr[kTrPrev]) {
+            st ^= kStateParity;
+            tr[kTrPrev] = d;
+            tr[kTrFieldNorm] = sqrt(norms[2 * b + 1]);
+        } else {
+            const double ai = tr[kTrAlphaI];
+            const double ai_new = 1.5 * ai;
+            const double A = ai / ai_new;
+            const double B = (ai_new - ai) / ai_new;  // i (alpha_old - alpha_new) / alpha_i,new is real here
+            tr[kTrA] *= A;
+            tr[kTrCr] = tr[kTrCr] * A + B;
+            tr[kTrCi] *= A;
+            tr[kTrAlphaI] = ai_new;
+            tr[kTrRejected] += 1.0;
+            Coef<T> c;
+            c.alpha_r = T(tr[kTrAlphaR]);
+            c.alpha_i = T(ai_new);
+            c.a = T(tr[kTrA]);
+            c.c_r = T(tr[kTrCr]);
+            c.c_i = T(tr[kTrCi]);
+            c.pad_ = T(0);
+            coef[b] = c;
+            st |= kStateStale;
+        }
+        const double prev = tr[kTrPrev];
+        const double res = prev > 0.0 ? prev / tr[kTrFieldNorm] : 0.0;
+        tr[kTrResidue] = res;
+        if (!(res > tol && it < max_iter)) {
+            st |= kStateDone;
+            tr[kTrDoneClock] = double(wall_clock64());
+        }
+        state[b] = st;
+    }
+}
+
+__global__ void wall_clock_kernel(double* out) { out[0] = double(wall_clock64()); }
 
 // sum |x|^2 in FP64: per-block partials (pass 1), then reduce_partials-style sum (pass 2).
 template <typename T>

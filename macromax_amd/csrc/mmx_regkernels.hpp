@@ -127,6 +127,18 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     const uint32_t toff = rows ? (uint32_t)(p * N + t) * ES : ((uint32_t)p + (uint32_t)t * (uint32_t)a.inner) * ES;
     const uint32_t mstep = rows ? (uint32_t)TP * ES : (uint32_t)TP * (uint32_t)a.inner * ES;
     const int64_t pb = b * a.bstride;
+    const cx<T>* Ecur = a.E;
+    cx<T>* Enext = a.Eout;
+    if (a.state != nullptr) {
+        // single-component passes hold pencils of one problem per tile (strided tiles never span problems)
+        constexpr bool single = (KIND == kFwd || KIND == kInv);
+        const unsigned st = a.state[single ? fdiv64(rows ? r0 : o, a.frpp) : b];
+        if ((st & kStateDone) || (a.only_stale && !(st & kStateStale))) return;
+        if (st & kStateParity) {
+            Ecur = a.Eout;
+            Enext = const_cast<cx<T>*>(a.E);
+        }
+    }
 
     for (int q = tid; q < N; q += blockDim.x) tw[q] = a.tw[q];
     __syncthreads();
@@ -135,7 +147,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     const cx<T> zero{T(0), T(0)};
     if constexpr (F::LOAD_PRO) {
         const int64_t V = a.comp_stride;
-        const cx<T>* Et = a.E + b * a.E_bstride + tb;
+        const cx<T>* Et = Ecur + b * a.E_bstride + tb;
         const cx<T>* st = a.s + b * a.s_bstride + tb;
         const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
 #pragma unroll
@@ -204,10 +216,10 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 
     if constexpr (F::EPI) {
         const int64_t V = a.comp_stride;
-        const cx<T>* Et = a.E + b * a.E_bstride + tb;
+        const cx<T>* Et = Ecur + b * a.E_bstride + tb;
         const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
         const cx<T>* st = a.s + b * a.s_bstride + tb;
-        cx<T>* Eot = a.Eout + b * a.E_bstride + tb;
+        cx<T>* Eot = Enext + b * a.E_bstride + tb;
         double dd = 0.0, ee = 0.0;
         if (valid) {
             // Loads and stores retire in issue order on this hardware, so every input of voxel m + 1 (E, chi and,

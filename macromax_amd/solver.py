@@ -490,6 +490,39 @@ class Solution:
         self.__update_operators(alpha_new)  # also marks the speculative first pass in the work buffer stale
         return False
 
+    # ------------------------------------------------------------------ batched solves (macromax_amd.batch)
+    def _batch_operands(self) -> dict:
+        """Device operands for a batched solve: chi and s for the set-up alpha, the affine coefficients, the
+        current field and the iteration bookkeeping."""
+        return {'chi': self.__chi, 'chi_kind': self.__chi_kind, 's': self.__source_normalized,
+                'coef': self.__coef, 'field': self.__field_array, 'alpha': self.__alpha, 'alpha_base': self.__alpha_base,
+                'beta': self.__beta, 'coef_a': self.__coef_a, 'coef_c': self.__coef_c,
+                'prev': self.__previous_update_norm, 'iteration': self.__iteration,
+                'ncomp': 1 + 2 * self.vectorial}
+
+    def _batch_source(self, source_distribution) -> torch.Tensor:
+        """s for the set-up alpha of this medium and another source distribution S, [C, 1, *grid]."""
+        BE = self.__BE
+        if not isinstance(source_distribution, torch.Tensor):
+            source_distribution = np.asarray(source_distribution).astype(BE.numpy_dtype)
+        S = BE.to_matrix_field(source_distribution)
+        full = (1 + 2 * self.vectorial, 1, *[int(_) for _ in self.grid.shape])
+        S = S.to(torch.complex128).expand(*full)
+        return (S * (1.0j / self.__alpha_base.imag / self.__beta)).to(BE.hardware_dtype)
+
+    def _batch_result(self, field: torch.Tensor, iteration: int, prev: float, field_norm: float, alpha: complex,
+                      coef_a: float, coef_c: complex):
+        """Take over the state a batched solve reached for this problem."""
+        self.__field_array.copy_(field)
+        self.__iteration = int(iteration)
+        self.__previous_update_norm = float(prev)
+        self.__field_norm = float(field_norm) if field_norm > 0 else None
+        self.__relative_residue = None
+        self.__alpha = complex(alpha)
+        self.__coef_a = float(coef_a)
+        self.__coef_c = complex(coef_c)
+        self.__upload_coef()
+
     def __iter__(self):
         self.iteration = 0
         while True:
