@@ -93,6 +93,129 @@ def cpu_baseline(sol, n):
             'os_cpu_count': os.cpu_count(), 'affinity_cores': len(os.sched_getaffinity(0))}
 
 
+# ---------------------------------------------------------------------------------------------------------------
+# Batched form: 2D scalar disc-scatterer problems (125 nm pixels, 500 nm wavelength, 32-pixel LinearBound with
+# kappa up to 0.2, line source on row 40), complex64, stopped at residual 1e-6.
+def disc_medium(n, seed, n_discs=40, pitch=125e-9, bound_px=32):
+    """Refractive index n x n: background 1 with n_discs discs (radius 0.5-2 um, index 1.2-1.5, later discs on top)
+    whose centres lie in the interior outside the absorbing layer."""
+    rng = np.random.default_rng(seed)
+    idx = np.ones((n, n))
+    x = np.arange(n) * pitch
+    xx, yy = np.meshgrid(x, x, indexing='ij')
+    lo, hi = bound_px * pitch, (n - bound_px) * pitch
+    for _ in range(n_discs):
+        cx, cy = rng.uniform(lo, hi, 2)
+        r = rng.uniform(0.5e-6, 2e-6)
+        idx[(xx - cx) ** 2 + (yy - cy) ** 2 < r ** 2] = rng.uniform(1.2, 1.5)
+    return idx
+
+
+def disc_problem(n, seed, source=None, pitch=125e-9, wavelength=500e-9, bound_px=32):
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    grid = mm.Grid(shape=[n, n], step=pitch)
+    idx = disc_medium(n, seed, pitch=pitch, bound_px=bound_px)
+    if source is None:
+        y = (np.arange(n) - n / 2) * pitch
+        source = np.zeros((1, n, n), dtype=np.complex64)
+        source[0, 40, :] = np.exp(-(y / 3e-6) ** 2)
+    bound = LinearBound(grid, thickness=bound_px * pitch, max_extinction_coefficient=0.2)
+    return mm.Solution(grid, vacuum_wavelength=wavelength, epsilon=idx ** 2, current_density=source, bound=bound,
+                       dtype=np.complex64)
+
+
+def plane_wave_sources(n, nrhs, pitch=125e-9, wavelength=500e-9):
+    k0 = 2 * np.pi / wavelength
+    y = (np.arange(n) - n / 2) * pitch
+    src = np.zeros((nrhs, 1, n, n), dtype=np.complex64)
+    for j, th in enumerate(np.linspace(-np.pi / 3, np.pi / 3, nrhs)):
+        src[j, 0, 40, :] = np.exp(1j * k0 * np.sin(th) * y)
+    return src
+
+
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def _rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs=64, steps_per_sync=16,
+                      log=lambda m: None):
+    """Part A: n_problems independent media with sizes drawn from {128,192,256,384,512}^2; Part B: one 512^2
+    medium with nrhs plane-wave line sources at angles in [-60, 60] degrees. Batched (solve_many /
+    BatchSolution, graph replays) against sequential Solution.solve, solves/s and per-problem latency."""
+    import torch
+    from macromax_amd.batch import BatchSolution, solve_many
+    cb = lambda s_: s_.residue > tol and s_.iteration < max_iter
+    out = {}
+    # Part A
+    sizes = np.random.default_rng(3).choice([128, 192, 256, 384, 512], n_problems)
+    seq = [disc_problem(int(n), 1000 + i) for i, n in enumerate(sizes)]
+    seq_t = []
+    for s in seq:
+        _, dt = _timed(lambda: s.solve(cb))
+        seq_t.append(dt)
+    log(f'part A sequential: {sum(seq_t):.2f} s')
+    bat = [disc_problem(int(n), 1000 + i) for i, n in enumerate(sizes)]
+    batches, bt_total = _timed(lambda: solve_many(bat, tol=tol, max_iter=max_iter, steps_per_sync=steps_per_sync))
+    log(f'part A batched: {bt_total:.2f} s')
+    lat = np.concatenate([b.latency() for b in batches])
+    errs = [_rel_l2(b.E, s.E) for b, s in zip(bat, seq)]
+    its = np.array([s.iteration for s in bat])
+    out['part_a'] = {
+        'problems': int(n_problems), 'sizes': {str(k): int(v) for k, v in zip(*np.unique(sizes, return_counts=True))},
+        'solves_per_s': round(n_problems / bt_total, 2), 'sequential_solves_per_s': round(n_problems / sum(seq_t), 2),
+        'speedup_vs_sequential': round(sum(seq_t) / bt_total, 2),
+        'latency_p50_s': round(float(np.percentile(lat, 50)), 4), 'latency_p99_s': round(float(np.percentile(lat, 99)), 4),
+        'sequential_latency_p50_s': round(float(np.percentile(seq_t, 50)), 4),
+        'sequential_latency_p99_s': round(float(np.percentile(seq_t, 99)), 4),
+        'iterations_median': int(np.median(its)), 'iterations_max': int(its.max()),
+        'iterations_equal_to_sequential': bool(all(a.iteration == b.iteration for a, b in zip(bat, seq))),
+        'max_rel_l2_vs_sequential': float(max(errs)),
+        'timed': 'solve_many end to end (batch set-up, graph capture, replays, write-back); sequential: '
+                 'Solution.solve per problem; medium set-up excluded from both'}
+    del seq, bat, batches
+    torch.cuda.empty_cache()
+    # Part B
+    n = 512
+    src = plane_wave_sources(n, nrhs)
+    medium = disc_problem(n, 0)
+    bt, bt_total = _timed(lambda: BatchSolution.from_sources(medium, current_densities=src).solve(
+        tol=tol, max_iter=max_iter, steps_per_sync=steps_per_sync))
+    log(f'part B batched: {bt_total:.2f} s')
+    Eb = bt.E
+    lat = bt.latency()
+    its_b = bt.iteration
+    seq_t, errs, its_eq = [], [], []
+    for j in range(min(seq_rhs, nrhs)):
+        s = disc_problem(n, 0, source=src[j])
+        _, dt = _timed(lambda: s.solve(cb))
+        seq_t.append(dt)
+        errs.append(_rel_l2(Eb[j], s.E))
+        its_eq.append(s.iteration == its_b[j])
+    log(f'part B sequential ({len(seq_t)} of {nrhs}): {sum(seq_t):.2f} s')
+    seq_rate = len(seq_t) / sum(seq_t)
+    out['part_b'] = {
+        'grid': [n, n], 'rhs': int(nrhs), 'solves_per_s': round(nrhs / bt_total, 2),
+        'sequential_solves_per_s': round(seq_rate, 2), 'speedup_vs_sequential': round(nrhs / bt_total / seq_rate, 2),
+        'sequential_measured_on': f'the first {len(seq_t)} of {nrhs} right-hand sides',
+        'latency_p50_s': round(float(np.percentile(lat, 50)), 4), 'latency_p99_s': round(float(np.percentile(lat, 99)), 4),
+        'sequential_latency_p50_s': round(float(np.percentile(seq_t, 50)), 4),
+        'sequential_latency_p99_s': round(float(np.percentile(seq_t, 99)), 4),
+        'iterations_median': int(np.median(its_b)), 'iterations_max': int(its_b.max()),
+        'iterations_equal_to_sequential': bool(all(its_eq)), 'max_rel_l2_vs_sequential': float(max(errs)),
+        'timed': 'BatchSolution.from_sources(...).solve end to end; sequential: Solution.solve per right-hand side'}
+    return out
+
+
 def measured_traffic(n):
     """HBM bytes per voxel-iteration from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
     path = os.path.join(ROOT, 'profiles', 'born_iterate_256cubed_aniso.md')
@@ -113,6 +236,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--residual', type=float, default=1e-5)
     ap.add_argument("--max-iter", type=int, default=10000)
+    ap.add_argument('--no-batched', action='store_true', help='skip the batched-solve comparison')
+    ap.add_argument('--batched-only', action='store_true', help='only the batched-solve comparison (own JSON line)')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='also run one device-to-device copy of this many MiB (calibrates HBM byte counters)')
     args = ap.parse_args()
@@ -122,6 +247,10 @@ def main():
         sys.exit(2)
     import torch
     dev = torch.device('cuda:0')
+    log = lambda m: print(f'bench: {m}', file=sys.stderr, flush=True)
+    if args.batched_only:
+        print(json.dumps({'batched': batched_benchmark(log=log)}), flush=True)
+        return
     n = args.grid
     V = n ** 3
     print(f'bench: building the {n}^3 anisotropic workload', file=sys.stderr, flush=True)
@@ -193,6 +322,10 @@ def main():
         cb = cpu_baseline(sol, n)
         out['cpu_baseline'] = cb
         out['cpu_time_to_residual_s_extrapolated'] = round(cb['seconds_per_iteration'] * iters_to_res, 1)
+    if not args.no_batched:
+        del sol
+        torch.cuda.empty_cache()
+        out['batched'] = batched_benchmark(log=log)
     print(json.dumps(out), flush=True)
 
 

@@ -18,6 +18,7 @@ from collections import OrderedDict
 from typing import Sequence
 
 import numpy as np
+import scipy.constants as const
 import torch
 
 from . import _native
@@ -102,13 +103,20 @@ class BatchSolution:
         self.__graph_key = None
 
     @classmethod
-    def from_sources(cls, solution: Solution, sources) -> 'BatchSolution':
+    def from_sources(cls, solution: Solution, sources=None, current_densities=None) -> 'BatchSolution':
         """Many right-hand sides of the medium of ``solution`` (its chi and alpha), all starting from E = 0.
 
         :param sources: source distributions S (as for ``Solution.source_distribution``), indexed by the first
             dimension: ``sources[b]`` is problem b's S.
+        :param current_densities: alternatively the current densities J (S = -i omega mu_0 J, solver.py:183).
         """
-        srcs = [sources[b] for b in range(len(sources))]
+        if (sources is None) == (current_densities is None):
+            raise ValueError('give either sources or current_densities')
+        if sources is None:
+            factor = -1j * solution.angular_frequency * const.mu_0
+            srcs = [np.asarray(current_densities[b]) * factor for b in range(len(current_densities))]
+        else:
+            srcs = [sources[b] for b in range(len(sources))]
         return cls([solution], _sources=srcs)
 
     # ------------------------------------------------------------------ properties

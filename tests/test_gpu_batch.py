@@ -7,6 +7,7 @@ differently shaped tiles in a batch, which can only move the accept decision at 
 """
 import numpy as np
 import pytest
+import scipy.constants as const
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +87,11 @@ def test_batch_of_sources_matches_sequential():
         r.solve(lambda s_: s_.residue > tol and s_.iteration < max_iter)
         assert bt.iteration[b] == r.iteration, b
         assert _rel(E[b], r.E) < 1e-5, b
+    # the same right-hand sides given as current densities J, S = -i omega mu_0 J
+    J = np.stack(srcs) / (-1j * base.angular_frequency * const.mu_0)
+    bj = BatchSolution.from_sources(base, current_densities=J).solve(tol=tol, max_iter=max_iter, steps_per_sync=5)
+    assert np.array_equal(bj.iteration, bt.iteration)
+    assert _rel(bj.E, E) < 1e-5
 
 
 def test_batch_rejections_match_sequential():
