@@ -23,6 +23,11 @@ oracle function                         reference
 ``chi_mul``                             ``backend/__init__.py:415-444`` (einsum 'ij...,jk...->ik...')
 ``BornOracle.step``                     ``solver.py:1064-1098`` (one __iter__ cycle incl. divergence guard)
 ``BornOracle.relative_residue``         ``solver.py:1026-1042``
+``solve_to``                            ``solver.py:1100-1123`` (Solution.solve with a residue callback)
+``transposed_k_vectors``                no reference counterpart: k vectors of the transposed operator
+``adjoint_gradient``                    no reference counterpart: reciprocity adjoint of the converged field
+                                        (derivation in its docstring), pinned by finite differences and by
+                                        autograd through ``oracle/born_torch.py``
 ``stockham_fft``                        model of the GPU kernels' Stockham pass schedule (no reference
                                         counterpart: the reference calls numpy.fft / torch.fft)
 =====================================  =====================================================================
@@ -40,6 +45,7 @@ __all__ = [
     'k_vectors', 'linear_bound_extinction', 'bound_susceptibility', 'sigma_ee', 'find_alpha_beta',
     'increase_bias', 'chi_base', 'normalized_source', 'longitudinal_projection_ft', 'green_op',
     'chi_mul', 'BornOracle', 'stockham_fft', 'plan_radices', 'mat3_max_eig_hermitian', 'setup_problem',
+    'solve_to', 'adjoint_gradient', 'transposed_k_vectors',
 ]
 
 
@@ -282,6 +288,55 @@ class BornOracle:
         if self.previous_update_norm > 0:
             return self.previous_update_norm / float(np.linalg.norm(self.E.ravel()))
         return 0.0
+
+
+def solve_to(chi, s, alpha, beta, k, vectorial, tol=1e-12, max_iter=20000, E0=None):
+    """Iterate ``BornOracle`` until ``not (residue > tol and iteration < max_iter)`` (``solver.py:1100-1123``)."""
+    o = BornOracle(chi, s, alpha, beta, k, vectorial, E0=E0)
+    while True:
+        o.step()
+        if not (o.relative_residue > tol and o.iteration < max_iter):
+            return o
+
+
+def transposed_k_vectors(k):
+    """k vectors of the transposed operator: the Nyquist entry of every even-length axis negated.
+
+    With A = F^-1 D(k) F (F the DFT matrix, symmetric), A^T = F^-1 D(k[-n]) F: the multiplier at the reflected
+    index. k[-n] = -k[n] except at the Nyquist index n = N/2 of an even axis, where fftfreq stores -pi/d for both,
+    so the dyadic cross terms k_i k_j / k^2 change sign on those planes. Negating the Nyquist entries gives the
+    multiplier of A^T (D is even in the whole k vector); the scalar Green's function only sees k^2 and is unchanged.
+    """
+    out = []
+    for kk in k:
+        kk = np.array(kk, dtype=float)
+        if kk.size % 2 == 0 and kk.size > 1:
+            kk[kk.size // 2] = -kk[kk.size // 2]
+        out.append(kk)
+    return out
+
+
+def adjoint_gradient(chi, alpha, beta, k, vectorial, E, g, tol=1e-12, max_iter=20000):
+    """d loss / d eps for a real loss of the converged (k0^2-scaled) field E, by one adjoint solve.
+
+    No reference counterpart (the reference has no gradient; ``examples/scattering_neural_network.py:206-318``
+    differentiates an unrolled scalar model). The converged field solves L E = S / beta with
+    L = curl curl (or -laplacian) - eps / beta (``solver.py:566-584`` with ``:480-484``, ``:702-720``). For
+    eps^T = eps, L^T is L with the Nyquist entries of k negated (``transposed_k_vectors``). With
+    g = dL/dRe(E) + i dL/dIm(E): dL = Re <g, dE> and dE = L^-1 (d eps E / beta), so mu = L^-T conj(g) is a forward
+    solve of the transposed operator with S' = beta conj(g) (s' = i conj(g) / alpha_i), and
+    dL/dRe(eps) + i dL/dIm(eps) = conj(sum_c mu_c E_c) / beta (isotropic eps) or conj(mu_i E_j) / beta (3x3).
+    Returns (gradient, adjoint oracle).
+    """
+    s_adj = np.conj(np.asarray(g, dtype=np.complex128)) * (1.0j / alpha.imag)
+    o = solve_to(chi, s_adj, alpha, beta, transposed_k_vectors(k), vectorial, tol, max_iter)
+    mu = o.E
+    E = np.asarray(E, dtype=np.complex128)
+    if chi.shape[0] == 1:
+        grad = np.conj(np.sum(mu * E, axis=0, keepdims=True)) / beta  # [1,1,*shape]
+    else:
+        grad = np.conj(mu[:, None, 0] * E[None, :, 0]) / beta  # [3,3,*shape]
+    return grad, o
 
 
 def setup_problem(shape, pitch, wavenumber, eps, current_density=None, source_distribution=None,
