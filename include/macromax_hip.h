@@ -57,6 +57,10 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *                        4 inverse+update, 5 single-axis): those passes give each thread twice the elements
  *   MMX_PARAM_REG_FFT    1 (default): complex64 axes of 64, 128, 256 or 512 points use the register-resident
  *                        FFT kernels; 0: every axis uses the LDS-staged kernels
+ *   MMX_PARAM_TRANSPOSED 0 (default): the operator of the reference (k = 2 pi fftfreq, utils/ft/grid.py:426-439);
+ *                        1: its transpose, for adjoint solves by reciprocity: the Nyquist entry of every
+ *                        even-length axis is +pi/d instead of -pi/d (only the dyadic Green's cross terms
+ *                        k_i k_j / k^2 on those planes change). Synchronises the device when it changes.
  */
 #define MMX_PARAM_G_AXIS 1
 #define MMX_PARAM_P_ROWS 2
@@ -66,6 +70,7 @@ int mmx_plan_destroy(mmx_plan_t plan);
 #define MMX_PARAM_LDS_BUDGET 6
 #define MMX_PARAM_BIG_ELEMS 7
 #define MMX_PARAM_REG_FFT 8
+#define MMX_PARAM_TRANSPOSED 9
 int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
 int mmx_plan_get_param(mmx_plan_t plan, int param, int64_t* value);
 

@@ -235,6 +235,7 @@ class BackEndHip(BackEnd):
         except OSError as e:
             raise ImportError(f'libmacromax_hip.so could not be loaded: {e}') from e
         self.__plan = Plan(grid.shape, grid.step, hardware_dtype, self.__device)
+        self.__transposed_plan = None
         self.__k = None
         self.__k2 = None
         self.__eye = None
@@ -247,6 +248,15 @@ class BackEndHip(BackEnd):
     @property
     def plan(self) -> Plan:
         return self.__plan
+
+    @property
+    def transposed_plan(self) -> Plan:
+        """A plan of the transposed operator (adjoint solves by reciprocity), created on first use."""
+        if self.__transposed_plan is None:
+            grid = self.grid
+            self.__transposed_plan = Plan(grid.shape, grid.step, self.hardware_dtype, self.__device)
+            self.__transposed_plan.set_param('transposed', 1)
+        return self.__transposed_plan
 
     @property
     def numpy_dtype(self):

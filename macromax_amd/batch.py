@@ -35,7 +35,7 @@ class BatchSolution:
         medium tensor (``from_sources``), otherwise chi is stacked per problem.
     """
 
-    def __init__(self, solutions: Sequence[Solution], _sources=None):
+    def __init__(self, solutions: Sequence[Solution], _sources=None, _transpose=False):
         sols = list(solutions)
         if len(sols) == 0:
             raise ValueError('a batch needs at least one problem')
@@ -57,7 +57,7 @@ class BatchSolution:
                     raise ValueError('all problems of a batch must share dtype, polarisation and chi storage')
         self.__solutions = sols
         self.__backend = first.backend
-        self.__plan = first.backend.plan
+        self.__plan = first.backend.transposed_plan if _transpose else first.backend.plan
         self.__shape = shape
         self.__wavenumber = first.wavenumber
         B = len(ops)
@@ -70,6 +70,8 @@ class BatchSolution:
         shared = self.__sources_mode or all(o['chi'] is ops0['chi'] for o in ops)
         if shared:
             self.__chi = ops0['chi']
+            if _transpose and ops0['chi_kind'] == 2:  # kinds 1 and 3 are symmetric
+                self.__chi = self.__chi.transpose(0, 1).contiguous()
             self.__chi_bstride = 0
         else:
             self.__chi = torch.stack([o['chi'] for o in ops]).contiguous()
@@ -103,12 +105,15 @@ class BatchSolution:
         self.__graph_key = None
 
     @classmethod
-    def from_sources(cls, solution: Solution, sources=None, current_densities=None) -> 'BatchSolution':
+    def from_sources(cls, solution: Solution, sources=None, current_densities=None,
+                     transpose: bool = False) -> 'BatchSolution':
         """Many right-hand sides of the medium of ``solution`` (its chi and alpha), all starting from E = 0.
 
         :param sources: source distributions S (as for ``Solution.source_distribution``), indexed by the first
             dimension: ``sources[b]`` is problem b's S.
         :param current_densities: alternatively the current densities J (S = -i omega mu_0 J, solver.py:183).
+        :param transpose: solve with the transposed operator (chi^T and the transposed Green's function), as an
+            adjoint solve by reciprocity needs.
         """
         if (sources is None) == (current_densities is None):
             raise ValueError('give either sources or current_densities')
@@ -117,7 +122,7 @@ class BatchSolution:
             srcs = [np.asarray(current_densities[b]) * factor for b in range(len(current_densities))]
         else:
             srcs = [sources[b] for b in range(len(sources))]
-        return cls([solution], _sources=srcs)
+        return cls([solution], _sources=srcs, _transpose=transpose)
 
     # ------------------------------------------------------------------ properties
     @property

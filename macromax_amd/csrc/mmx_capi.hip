@@ -55,6 +55,8 @@ struct mmx_plan_s {
     int naxes;
     AxisPlan ax[3];  // non-trivial axes, outermost first
     void* kvec;      // device k-vectors, n0 + n1 + n2 reals
+    double step3[3]; // grid step per axis (1 for padded axes)
+    int transposed;  // 1: k-vectors of the transposed operator (Nyquist entries negated)
     int k1ofs, k2ofs;
     void* scratch;   // FP64 partial sums
     size_t scratch_bytes;
@@ -90,6 +92,32 @@ int upload_real(const std::vector<double>& h, void** dev) {
     std::vector<T> tmp(h.begin(), h.end());
     MMX_HIP_CHECK(hipMalloc(dev, tmp.size() * sizeof(T)));
     MMX_HIP_CHECK(hipMemcpy(*dev, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return MMX_OK;
+}
+
+// k-vectors 2 pi fftfreq(n, d) in the order of numpy.fft.fftfreq (utils/ft/grid.py:426-439). For the transposed
+// operator the Nyquist entry -pi/d of every even axis becomes +pi/d: A = F^-1 D(k) F has A^T = F^-1 D(k[-n]) F, and
+// k[-n] = -k[n] everywhere except at n = N/2, where the dyadic Green's cross terms change sign.
+std::vector<double> k_vectors(const mmx_plan_s* p) {
+    std::vector<double> k;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t n = p->shape3[a];
+        const double val = 1.0 / (double(n) * p->step3[a]);
+        const int64_t npos = (n - 1) / 2 + 1;
+        for (int64_t i = 0; i < n; ++i) {
+            double f = (i < npos ? double(i) : double(i - n)) * val;
+            if (p->transposed && n % 2 == 0 && n > 1 && i == n / 2) f = -f;
+            k.push_back(2.0 * M_PI * f);
+        }
+    }
+    return k;
+}
+
+template <typename T>
+int reupload_real(const std::vector<double>& h, void* dev) {
+    std::vector<T> tmp(h.begin(), h.end());
+    MMX_HIP_CHECK(hipDeviceSynchronize());
+    MMX_HIP_CHECK(hipMemcpy(dev, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
     return MMX_OK;
 }
 
@@ -803,20 +831,11 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
         int rc = dtype == MMX_C64 ? make_twiddles<float>(ax.N, &ax.tw) : make_twiddles<double>(ax.N, &ax.tw);
         if (rc) { mmx_plan_destroy(p); return rc; }
     }
-    // k-vectors: 2 pi fftfreq(n, d) in the order of numpy.fft.fftfreq
-    std::vector<double> k;
-    for (int a = 0; a < 3; ++a) {
-        const int64_t n = p->shape3[a];
-        const double d = a < ndim ? step[a] : 1.0;
-        const double val = 1.0 / (double(n) * d);
-        const int64_t npos = (n - 1) / 2 + 1;
-        for (int64_t i = 0; i < n; ++i) {
-            const double f = (i < npos ? double(i) : double(i - n)) * val;
-            k.push_back(2.0 * M_PI * f);
-        }
-    }
+    for (int a = 0; a < 3; ++a) p->step3[a] = a < ndim ? step[a] : 1.0;
+    p->transposed = 0;
     p->k1ofs = (int)p->shape3[0];
     p->k2ofs = (int)(p->shape3[0] + p->shape3[1]);
+    const std::vector<double> k = k_vectors(p);
     int rc = dtype == MMX_C64 ? upload_real<float>(k, &p->kvec) : upload_real<double>(k, &p->kvec);
     if (rc) { mmx_plan_destroy(p); return rc; }
     *out = p;
@@ -849,6 +868,17 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
         case MMX_PARAM_LDS_BUDGET: p->lds_budget = value ? (size_t)value : kLdsBudget; break;
         case MMX_PARAM_BIG_ELEMS: p->big_elems = (int)value; break;
         case MMX_PARAM_REG_FFT: p->reg_fft = value ? 1 : 0; break;
+        case MMX_PARAM_TRANSPOSED: {
+            const int t = value ? 1 : 0;
+            if (t != p->transposed) {
+                p->transposed = t;
+                const std::vector<double> k = k_vectors(p);
+                const int rc = p->dtype == MMX_C64 ? reupload_real<float>(k, p->kvec)
+                                                   : reupload_real<double>(k, p->kvec);
+                if (rc) return rc;
+            }
+            break;
+        }
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;
@@ -866,6 +896,7 @@ int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
         case MMX_PARAM_LDS_BUDGET: *value = (int64_t)p->lds_budget; break;
         case MMX_PARAM_BIG_ELEMS: *value = p->big_elems; break;
         case MMX_PARAM_REG_FFT: *value = p->reg_fft; break;
+        case MMX_PARAM_TRANSPOSED: *value = p->transposed; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;

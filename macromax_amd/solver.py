@@ -500,6 +500,11 @@ class Solution:
                 'prev': self.__previous_update_norm, 'iteration': self.__iteration,
                 'ncomp': 1 + 2 * self.vectorial}
 
+    def _set_alpha(self, alpha: complex):
+        """Use the preconditioner constant ``alpha`` from now on (chi and s rescaled as after a rejected update)."""
+        if complex(alpha) != self.__alpha:
+            self.__update_operators(complex(alpha))
+
     def _batch_source(self, source_distribution) -> torch.Tensor:
         """s for the set-up alpha of this medium and another source distribution S, [C, 1, *grid]."""
         BE = self.__BE
