@@ -216,6 +216,87 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
     return out
 
 
+# ---------------------------------------------------------------------------------------------------------------
+# Adjoint gradient: 96^3 isotropic vectorial, complex64; design cube 48^3 (eps uniform in [1, 2.25], seed 5),
+# y-polarised Gaussian current on plane z = 12, loss = sum |E|^2 over a 6^3 box centred 6 pixels behind the design.
+def adjoint_benchmark(n=96, design=48, tol=1e-6, max_iter=10000, n_fd=50, h=1e-3, log=lambda m: None):
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.adjoint import solve_field, fields_at_fixed_alpha
+    from macromax_amd.bound import LinearBound
+    pitch, wavelength = 250e-9, 1e-6
+    grid = mm.Grid(shape=[n, n, n], step=pitch)
+    bound = LinearBound(grid, thickness=16 * pitch)
+    eps = np.ones((n, n, n))
+    d0 = (n - design) // 2
+    dsl = (slice(d0, d0 + design),) * 3
+    eps[dsl] = np.random.default_rng(5).uniform(1.0, 2.25, (design,) * 3)
+    x = (np.arange(n) - n / 2) * pitch
+    J = np.zeros((3, n, n, n), dtype=np.complex64)
+    J[1, :, :, 12] = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2e-6) ** 2)
+    zc = d0 + design + 6
+    box = (slice(n // 2 - 3, n // 2 + 3), slice(n // 2 - 3, n // 2 + 3), slice(zc - 3, zc + 3))
+
+    def loss_of(E):
+        return torch.sum(torch.abs(E[(slice(None),) + box]) ** 2)
+
+    def gradient():
+        eps_t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)
+        info = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = solve_field(eps_t, grid, wavelength, current_density=J, bound=bound, tol=tol, max_iter=max_iter,
+                        info=info)
+        loss = loss_of(E)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        return eps_t.grad.numpy(), info, t_fwd, time.perf_counter() - t0
+
+    _, _, t_fwd0, t_bwd0 = gradient()  # first call in the process: includes one-time start-up costs
+    grad, info, t_fwd, t_bwd = gradient()
+    log(f'adjoint: forward {t_fwd:.3f} s ({info["forward_iterations"]} it), backward {t_bwd:.3f} s '
+        f'({info["adjoint_iterations"]} it)')
+    # forward solve alone (set-up excluded), for the ratio of iteration work
+    sol = mm.Solution(grid, vacuum_wavelength=wavelength, epsilon=eps, current_density=J, bound=bound,
+                      dtype=np.complex64)
+    _, t_iter = _timed(lambda: sol.solve(lambda s_: s_.residue > tol and s_.iteration < max_iter))
+    # central finite differences at frozen alpha on n_fd random voxels of the design region
+    rng = np.random.default_rng(6)
+    vox = [tuple(int(d0 + rng.integers(0, design)) for _ in range(3)) for _ in range(n_fd)]
+    epsilons = []
+    for v in vox:
+        for sgn in (+1, -1):
+            e = eps.copy()
+            e[v] += sgn * h
+            epsilons.append(e)
+    Efd, t_fd = _timed(lambda: fields_at_fixed_alpha(epsilons, grid, wavelength, info['alpha'], current_density=J,
+                                                     bound=bound, tol=tol, max_iter=max_iter))
+    losses = [float(loss_of(Efd[i])) for i in range(len(epsilons))]
+    fd = np.array([(losses[2 * i] - losses[2 * i + 1]) / (2 * h) for i in range(n_fd)])
+    ad = np.array([grad[v] for v in vox])
+    gmax = float(np.abs(grad[dsl]).max())
+    log(f'adjoint: {2 * n_fd} finite-difference solves in one batch, {t_fd:.2f} s')
+    return {
+        'grid': [n, n, n], 'design': [design] * 3, 'dtype': 'c64', 'tol': tol,
+        'forward_s': round(t_fwd, 4), 'backward_s': round(t_bwd, 4),
+        'first_call_forward_s': round(t_fwd0, 4), 'first_call_backward_s': round(t_bwd0, 4),
+        'backward_over_forward': round(t_bwd / t_fwd, 3),
+        'forward_solve_only_s': round(t_iter, 4), 'backward_over_forward_solve_only': round(t_bwd / t_iter, 3),
+        'forward_iterations': int(info['forward_iterations']), 'adjoint_iterations': int(info['adjoint_iterations']),
+        'fd_voxels': n_fd, 'fd_h': h,
+        'fd_max_abs_err_over_max_grad': float(np.abs(ad - fd).max() / gmax),
+        'fd_median_rel_err': float(np.median(np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-300))),
+        'fd_max_rel_err_where_grad_above_10pct': float(np.max(np.where(np.abs(ad) > 0.1 * gmax,
+                                                                        np.abs(ad - fd) / np.abs(ad), 0.0))),
+        'fd_batch_s': round(t_fd, 3),
+        'timed': 'second forward/backward in the process (the first, with start-up costs, is first_call_*): '
+                 'forward = Solution set-up + solve + loss; backward = adjoint solve of the transposed operator + '
+                 'gradient assembly (loss.backward()); forward_solve_only = Solution.solve without set-up'}
+
+
 def measured_traffic(n):
     """HBM bytes per voxel-iteration from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
     path = os.path.join(ROOT, 'profiles', 'born_iterate_256cubed_aniso.md')
@@ -238,6 +319,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=10000)
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-solve comparison')
     ap.add_argument('--batched-only', action='store_true', help='only the batched-solve comparison (own JSON line)')
+    ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient measurement')
+    ap.add_argument('--adjoint-only', action='store_true', help='only the adjoint-gradient measurement')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='also run one device-to-device copy of this many MiB (calibrates HBM byte counters)')
     args = ap.parse_args()
@@ -250,6 +333,9 @@ def main():
     log = lambda m: print(f'bench: {m}', file=sys.stderr, flush=True)
     if args.batched_only:
         print(json.dumps({'batched': batched_benchmark(log=log)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({'adjoint': adjoint_benchmark(log=log)}), flush=True)
         return
     n = args.grid
     V = n ** 3
@@ -326,6 +412,9 @@ def main():
         del sol
         torch.cuda.empty_cache()
         out['batched'] = batched_benchmark(log=log)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        out['adjoint'] = adjoint_benchmark(log=log)
     print(json.dumps(out), flush=True)
 
 

@@ -69,7 +69,10 @@ class BornField(torch.autograd.Function):
         # S' = beta conj(g), normalised to unit norm (the solve is linear; the residue test is relative)
         src = (torch.conj(g) * (beta / scale)).reshape(1, C, *g.shape[1:])
         tol, max_iter = problem.get('tol', 1e-6), problem.get('max_iter', 10000)
-        adj = BatchSolution.from_sources(sol, sources=src, transpose=True).solve(tol=tol, max_iter=max_iter)
+        # one problem: eager launches (16 iterations between host checks) cost the same as graph replays here and
+        # avoid a capture per backward pass
+        adj = BatchSolution.from_sources(sol, sources=src, transpose=True).solve(tol=tol, max_iter=max_iter,
+                                                                                graph=False)
         problem['adjoint_iterations'] = int(adj.iteration[0])
         problem['adjoint_residue'] = float(adj.residue[0])
         mu = adj.field_arrays()[0] * scale  # [C, 1, *grid], k0^2-scaled
