@@ -72,6 +72,13 @@ int mmx_plan_destroy(mmx_plan_t plan);
 #define MMX_PARAM_REG_FFT 8
 #define MMX_PARAM_TRANSPOSED 9
 int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
+/*
+ * Support of the sources s of the Born calls that follow on this plan: s is zero outside the box
+ * lo[a] <= i_a < hi[a] (a < ndim), so the passes that add s read it only inside. NULL, NULL restores the whole
+ * grid (the default). No reference counterpart: the reference stores and reads s densely (solver.py:702-720,
+ * 1070). A caller that changes s, or passes a batch with several sources, sets the box (their union) first.
+ */
+int mmx_plan_set_source_support(mmx_plan_t plan, const int64_t* lo, const int64_t* hi);
 int mmx_plan_get_param(mmx_plan_t plan, int param, int64_t* value);
 
 /*

@@ -17,7 +17,8 @@ MMX_C64 = 0
 MMX_C128 = 1
 
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
-            'mmx_plan_get_param', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue', 'mmx_born_iterate',
+            'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue',
+            'mmx_born_iterate',
             'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
 
@@ -51,6 +52,8 @@ def _declare(L):
     L.mmx_plan_destroy.argtypes = [c_p]
     L.mmx_plan_set_param.restype = c_int
     L.mmx_plan_set_param.argtypes = [c_p, c_int, c_i64]
+    L.mmx_plan_set_source_support.restype = c_int
+    L.mmx_plan_set_source_support.argtypes = [c_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)]
     L.mmx_plan_get_param.restype = c_int
     L.mmx_plan_get_param.argtypes = [c_p, c_int, ctypes.POINTER(c_i64)]
     L.mmx_fft.restype = c_int

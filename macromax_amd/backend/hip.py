@@ -97,6 +97,20 @@ class Plan:
         """Set a tuning parameter (see include/macromax_hip.h, MMX_PARAM_*); 0 = automatic."""
         _native.check(self._lib.mmx_plan_set_param(self.handle, _native.PARAMS[name], int(value)), 'mmx_plan_set_param')
 
+    def set_source_support(self, lo=None, hi=None):
+        """Box [lo, hi) outside which the sources of later Born calls are zero (None: the whole grid)."""
+        key = None if lo is None else (tuple(int(_) for _ in lo), tuple(int(_) for _ in hi))
+        if key == getattr(self, '_source_box', None):
+            return
+        if key is None:
+            rc = self._lib.mmx_plan_set_source_support(self.handle, None, None)
+        else:
+            lo_a = (ctypes.c_int64 * self.ndim)(*key[0])
+            hi_a = (ctypes.c_int64 * self.ndim)(*key[1])
+            rc = self._lib.mmx_plan_set_source_support(self.handle, lo_a, hi_a)
+        _native.check(rc, 'mmx_plan_set_source_support')
+        self._source_box = key
+
     def get_param(self, name: str) -> int:
         v = ctypes.c_int64()
         _native.check(self._lib.mmx_plan_get_param(self.handle, _native.PARAMS[name], ctypes.byref(v)), 'mmx_plan_get_param')
@@ -122,7 +136,8 @@ class Plan:
                                         _stream_ptr(self.device)), 'mmx_fft')
         return out
 
-    def born_step(self, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms):
+    def born_step(self, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
+                  s_box=None):
         for t, w in ((E, 'E'), (Eout, 'Eout'), (s, 's'), (chi, 'chi')):
             self._check(t, w)
         if coef.dtype != self.real_dtype or coef.device != self.device or coef.numel() < 6 * nbatch:
@@ -136,6 +151,7 @@ class Plan:
             raise ValueError('born_step: chi size mismatch')
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
             raise ValueError('born_step: s size mismatch')
+        self.set_source_support(*(s_box if s_box is not None else (None, None)))
         _native.check(self._lib.mmx_born_step(self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(),
                                               chi.data_ptr(), int(ncomp), int(chi_kind), coef.data_ptr(),
                                               int(nbatch), int(s_bstride), int(chi_bstride), norms.data_ptr(),
@@ -158,21 +174,25 @@ class Plan:
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
             raise ValueError('s size mismatch')
 
-    def born_prologue(self, E, W, s, chi, ncomp, chi_kind, coef, nbatch=1, s_bstride=0, chi_bstride=0):
+    def born_prologue(self, E, W, s, chi, ncomp, chi_kind, coef, nbatch=1, s_bstride=0, chi_bstride=0,
+                      s_box=None):
         """W = FFT along the innermost axis of chi_eff E + s_eff (``mmx_born_prologue``)."""
         self._born_checks(E, ((W, 'W'),), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride)
+        self.set_source_support(*(s_box if s_box is not None else (None, None)))
         _native.check(self._lib.mmx_born_prologue(self.handle, E.data_ptr(), W.data_ptr(), s.data_ptr(),
                                                   chi.data_ptr(), int(ncomp), int(chi_kind), coef.data_ptr(),
                                                   int(nbatch), int(s_bstride), int(chi_bstride),
                                                   _stream_ptr(self.device)), 'mmx_born_prologue')
         return W
 
-    def born_iterate(self, E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch=1, s_bstride=0, chi_bstride=0):
+    def born_iterate(self, E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch=1, s_bstride=0, chi_bstride=0,
+                     s_box=None):
         """Finish the iteration begun in W (``mmx_born_iterate``): Eout = E + d, norms, W = next prologue."""
         self._born_checks(E, ((Eout, 'Eout'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
                           chi_bstride)
         if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
             raise ValueError('norms must be a float64 device tensor of 2*nbatch elements')
+        self.set_source_support(*(s_box if s_box is not None else (None, None)))
         _native.check(self._lib.mmx_born_iterate(self.handle, E.data_ptr(), Eout.data_ptr(), W.data_ptr(),
                                                  s.data_ptr(), chi.data_ptr(), int(ncomp), int(chi_kind),
                                                  coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
@@ -180,7 +200,7 @@ class Plan:
         return norms
 
     def batch_step(self, EA, EB, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, state, track,
-                   norms, tol, max_iter):
+                   norms, tol, max_iter, s_box=None):
         """One device-decided iteration of every unfinished problem of a batch (``mmx_batch_step``)."""
         self._born_checks(EA, ((EB, 'EB'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
                           chi_bstride)
@@ -190,6 +210,7 @@ class Plan:
             raise ValueError('track must be a float64 device tensor of nbatch x TRACK_LEN elements')
         if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
             raise ValueError('norms must be a float64 device tensor of 2*nbatch elements')
+        self.set_source_support(*(s_box if s_box is not None else (None, None)))
         _native.check(self._lib.mmx_batch_step(self.handle, EA.data_ptr(), EB.data_ptr(), W.data_ptr(),
                                                s.data_ptr(), chi.data_ptr(), int(ncomp), int(chi_kind),
                                                coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
@@ -535,21 +556,24 @@ class BackEndHip(BackEnd):
 
     # ------------------------------------------------------------------ fused iteration
     def born_step(self, E, Eout, s, chi, chi_kind: int, coef, norms, nbatch: int = 1, s_bstride: int = 0,
-                  chi_bstride: int = 0):
-        """d = chi [G(chi E + s) - E]; Eout = E + d; norms = [sum |d|^2, sum |Eout|^2] (FP64, on the device)."""
+                  chi_bstride: int = 0, s_box=None):
+        """d = chi [G(chi E + s) - E]; Eout = E + d; norms = [sum |d|^2, sum |Eout|^2] (FP64, on the device).
+        ``s_box`` = (lo, hi): s is zero outside that box (None: anywhere)."""
         ncomp = E.shape[-self.grid.ndim - 2] if E.ndim >= self.grid.ndim + 2 else 1
-        return self.__plan.born_step(E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms)
+        return self.__plan.born_step(E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
+                                     s_box=s_box)
 
     def born_prologue(self, E, W, s, chi, chi_kind: int, coef, nbatch: int = 1, s_bstride: int = 0,
-                      chi_bstride: int = 0):
+                      chi_bstride: int = 0, s_box=None):
         ncomp = E.shape[-self.grid.ndim - 2]
-        return self.__plan.born_prologue(E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride)
+        return self.__plan.born_prologue(E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride,
+                                         s_box=s_box)
 
     def born_iterate(self, E, Eout, W, s, chi, chi_kind: int, coef, norms, nbatch: int = 1, s_bstride: int = 0,
-                     chi_bstride: int = 0):
+                     chi_bstride: int = 0, s_box=None):
         ncomp = E.shape[-self.grid.ndim - 2]
         return self.__plan.born_iterate(E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride,
-                                        chi_bstride)
+                                        chi_bstride, s_box=s_box)
 
     def __str__(self) -> str:
         return f'BackEndHip(vectorial={self.vectorial}, {self.grid}, dtype={self.numpy_dtype}, device={self.__device})'

@@ -22,7 +22,7 @@ import scipy.constants as const
 import torch
 
 from . import _native
-from .solver import Solution
+from .solver import Solution, source_support
 
 __all__ = ['BatchSolution', 'solve_many']
 
@@ -83,6 +83,17 @@ class BatchSolution:
             self.__s = torch.stack([o['s'] for o in ops]).contiguous()
             fields = torch.stack([o['field'] for o in ops]).contiguous()
         self.__s_bstride = C * V
+        # union of the problems' source supports (the passes read s only inside it)
+        if self.__sources_mode:
+            box = source_support(self.__s, len(shape))
+        else:
+            boxes = [o['s_box'] for o in ops if list(o['s_box'][0]) != list(o['s_box'][1])]
+            if boxes:
+                box = ([min(b[0][a] for b in boxes) for a in range(len(shape))],
+                       [max(b[1][a] for b in boxes) for a in range(len(shape))])
+            else:
+                box = ([0] * len(shape), [0] * len(shape))
+        self.__s_box = box
         self.__coef = torch.stack([o['coef'] for o in ops]).contiguous()
         self.__EA = fields
         self.__EB = torch.zeros_like(fields)
@@ -189,7 +200,7 @@ class BatchSolution:
     def _launch(self, tol: float, max_iter: int):
         self.__plan.batch_step(self.__EA, self.__EB, self.__W, self.__s, self.__chi, self.__ncomp, self.__chi_kind,
                                self.__coef, self.__nbatch, self.__s_bstride, self.__chi_bstride, self.__state,
-                               self.__track, self.__norms, tol, max_iter)
+                               self.__track, self.__norms, tol, max_iter, s_box=self.__s_box)
 
     def _prepare(self, tol: float, max_iter: int):
         """One launch with every problem marked finished: allocates the plan's scratch and sets kernel attributes

@@ -57,6 +57,7 @@ struct mmx_plan_s {
     void* kvec;      // device k-vectors, n0 + n1 + n2 reals
     double step3[3]; // grid step per axis (1 for padded axes)
     int transposed;  // 1: k-vectors of the transposed operator (Nyquist entries negated)
+    int64_t s_lo[3], s_hi[3];  // support box of the sources of later Born calls (default: the whole grid)
     int k1ofs, k2ofs;
     void* scratch;   // FP64 partial sums
     size_t scratch_bytes;
@@ -249,8 +250,16 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 
 // ---- register-resident kernels (complex64, power-of-two N) ----
 // points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels
+#ifndef MMX_E_FUSED3
+#define MMX_E_FUSED3 8
+#endif
+#ifndef MMX_E_GREEN3
+#define MMX_E_GREEN3 8
+#endif
 constexpr int reg_elems(int N, int NC, int KIND) {
-    return (KIND == kEpiPro || KIND == kGreenEpiPro) ? 8 : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
+    return (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
+           : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
+           : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
 }
 
 // Power-of-two lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the
@@ -260,16 +269,20 @@ bool reg_supported(const AxisPlan& ax) {
     return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
 }
 
-int reg_threads_per_pencil(int N, int NC, int kind) {
-    const bool fusedlast = (kind == kEpiPro || kind == kGreenEpiPro);
-    return N / reg_elems(N, NC, fusedlast ? kEpiPro : kFwd);
-}
+int reg_threads_per_pencil(int N, int NC, int kind) { return N / reg_elems(N, NC, kind); }
 
 template <int N, int KIND, int NC, int CK>
 void launch_reg(const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
     constexpr int E = reg_elems(N, NC, KIND);
-    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds,
-                       stream, a);
+    if constexpr (KIND == kEpiPro || KIND == kGreenEpiPro) {
+        if (a.s_restricted) {
+            hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 1>), dim3((unsigned)t.nblocks),
+                               dim3(t.nth), t.lds, stream, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 0>), dim3((unsigned)t.nblocks), dim3(t.nth),
+                       t.lds, stream, a);
 }
 
 template <int KIND, int NC, int CK>
@@ -525,6 +538,12 @@ PassArgs<T> born_base(const mmx_plan_s* p, const void* E, void* Eout, const void
     base.partials = partials;
     base.coef = reinterpret_cast<const Coef<T>*>(coef);
     base.vectorial = (C == 3);
+    base.s_restricted = 0;
+    for (int a = 0; a < 3; ++a) {
+        base.s_lo[a] = (int)p->s_lo[a];
+        base.s_hi[a] = (int)p->s_hi[a];
+        if (p->s_lo[a] != 0 || p->s_hi[a] != p->shape3[a]) base.s_restricted = 1;
+    }
     return base;
 }
 
@@ -833,6 +852,7 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
     }
     for (int a = 0; a < 3; ++a) p->step3[a] = a < ndim ? step[a] : 1.0;
     p->transposed = 0;
+    for (int a = 0; a < 3; ++a) { p->s_lo[a] = 0; p->s_hi[a] = p->shape3[a]; }
     p->k1ofs = (int)p->shape3[0];
     p->k2ofs = (int)(p->shape3[0] + p->shape3[1]);
     const std::vector<double> k = k_vectors(p);
@@ -899,6 +919,21 @@ int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
         case MMX_PARAM_TRANSPOSED: *value = p->transposed; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
+    return MMX_OK;
+}
+
+int mmx_plan_set_source_support(mmx_plan_t p, const int64_t* lo, const int64_t* hi) {
+    g_last_error.clear();
+    if (!p) return fail(MMX_EINVAL, "null plan");
+    if ((lo == nullptr) != (hi == nullptr)) return fail(MMX_EINVAL, "give both lo and hi, or neither");
+    int64_t l[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        l[a] = (lo && a < p->ndim) ? lo[a] : 0;
+        h[a] = (hi && a < p->ndim) ? hi[a] : p->shape3[a];
+        if (l[a] < 0 || h[a] > p->shape3[a] || l[a] > h[a]) return fail(MMX_EINVAL, "source box outside the grid");
+        if (h[a] >= (int64_t(1) << 31)) return fail(MMX_EINVAL, "source box too large");
+    }
+    for (int a = 0; a < 3; ++a) { p->s_lo[a] = l[a]; p->s_hi[a] = h[a]; }
     return MMX_OK;
 }
 

@@ -74,6 +74,10 @@ struct PassArgs {
     // last passes read the current field from E or Eout by the parity bit and write the next one to the other
     const unsigned* state;
     int only_stale;           // first-pass launch: only the problems whose work buffer is stale
+    // support of the source: s is zero outside [s_lo, s_hi) on every (padded) grid axis; s_restricted = 0 when the
+    // box is the whole grid. Passes that add s skip its loads outside the box.
+    int s_restricted;
+    int s_lo[3], s_hi[3];
     FDiv64 frpp;              // single-component passes: outer index o -> problem (o / (C * outer extent))
     // Green's function
     int64_t n0, n1, n2;       // grid shape padded with trailing 1s
@@ -123,6 +127,20 @@ __device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, b
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) f[c] = g * (f[c] - pl[c]) + gl * pl[c];
+}
+
+// Is the pencil with outer index `row` (the axes before the pass axis, C order, one component of one problem)
+// inside the source box on those axes? Passes that add s run along the innermost axis.
+template <typename T>
+__device__ __forceinline__ bool src_row_in_box(const PassArgs<T>& a, int64_t row) {
+    if (!a.s_restricted) return true;
+    if (a.axis_orig == 2) {
+        const int64_t i0 = fdiv64(row, a.fn1);
+        const int64_t i1 = row - i0 * a.n1;
+        return i0 >= a.s_lo[0] && i0 < a.s_hi[0] && i1 >= a.s_lo[1] && i1 < a.s_hi[1];
+    }
+    if (a.axis_orig == 1) return row >= a.s_lo[0] && row < a.s_hi[0];
+    return true;
 }
 
 template <typename T>
@@ -304,8 +322,11 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) if (c < a.C) x[c] = Eb[c * V + v];
                 chi_apply<T>(a, chib, v, x, y, cf);
+                const int ax = a.axis_orig;
+                if (!a.s_restricted || (j >= a.s_lo[ax] && j < a.s_hi[ax] && src_row_in_box(a, row))) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = y[c] + scal(sb[c * V + v], cf.a);
+                    for (int c = 0; c < 3; ++c) if (c < a.C) y[c] = y[c] + scal(sb[c * V + v], cf.a);
+                }
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) if (c < a.C) tile[(c * a.P + p) * g.SM + j] = y[c];
@@ -402,8 +423,13 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
                 const cx<T>* sb = a.s + b * a.s_bstride;
                 cx<T> x[3] = {z, z, z};
                 chi_mulv<T>(a, q, en, x, cf);
+                const int ax = a.axis_orig;
+                if (!a.s_restricted || (j >= a.s_lo[ax] && j < a.s_hi[ax] && src_row_in_box(a, row))) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) if (c < a.C) tile[(c * a.P + p) * g.SM + j] = x[c] + scal(sb[c * V + v], cf.a);
+                    for (int c = 0; c < 3; ++c) if (c < a.C) x[c] = x[c] + scal(sb[c * V + v], cf.a);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) if (c < a.C) tile[(c * a.P + p) * g.SM + j] = x[c];
             }
         }
         write_partials(dd, ee, a, blk, red);

@@ -82,7 +82,9 @@ __device__ __forceinline__ void epi_load(EpiIn<T, NC, WITH_S>& in, const cx<T>* 
 
 // The host launches this kernel only when (N * inner + P) * sizeof(cx<T>) < 2^32, so every offset within a tile
 // fits the 32-bit byte offsets used below. CK is the chi kind of the passes that apply chi (0 for the others).
-template <typename T, int N, int E, int KIND, int NC, int CK>
+// SB = 1 (update passes that also compute the next first pass, with a source confined to a box): s is added after
+// the update loop, only inside the box, instead of being loaded with every voxel's E and chi.
+template <typename T, int N, int E, int KIND, int NC, int CK, int SB>
 __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     using F = PassFlags<KIND>;
     constexpr int TP = N / E;
@@ -145,6 +147,17 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 
     cx<T> v[NC][E];
     const cx<T> zero{T(0), T(0)};
+    // source support of this pencil: rows inside the box, points j in [s_jlo, s_jhi) (passes that add s are row
+    // passes along the innermost axis)
+    bool s_row = true;
+    int s_jlo = 0, s_jhi = N;
+    if constexpr (F::LOAD_PRO || F::PRO_AFTER) {
+        if (a.s_restricted) {
+            s_row = src_row_in_box(a, r0 + p);
+            s_jlo = a.s_lo[a.axis_orig];
+            s_jhi = a.s_hi[a.axis_orig];
+        }
+    }
     if constexpr (F::LOAD_PRO) {
         const int64_t V = a.comp_stride;
         const cx<T>* Et = Ecur + b * a.E_bstride + tb;
@@ -160,8 +173,11 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
                 ChiVox<T> qv;
                 chi_load_ck<T, CK>(chit, V, bo, qv);
                 chi_mul_ck<T, CK, NC>(qv, x, y, cf);
+                const int j = t + m * TP;
+                if (s_row && j >= s_jlo && j < s_jhi) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(at(st + c * V, bo), cf.a);
+                    for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(at(st + c * V, bo), cf.a);
+                }
             }
 #pragma unroll
             for (int c = 0; c < NC; ++c) v[c][m] = y[c];
@@ -224,11 +240,12 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         if (valid) {
             // Loads and stores retire in issue order on this hardware, so every input of voxel m + 1 (E, chi and,
             // for the fused next prologue, s) is requested before voxel m's result is stored.
-            EpiIn<T, NC, F::PRO_AFTER> cur, nxt;
-            epi_load<T, NC, CK, F::PRO_AFTER>(cur, Et, chit, st, V, toff);
+            constexpr bool S_INLINE = F::PRO_AFTER && SB == 0;
+            EpiIn<T, NC, S_INLINE> cur, nxt;
+            epi_load<T, NC, CK, S_INLINE>(cur, Et, chit, st, V, toff);
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                if (m + 1 < E) epi_load<T, NC, CK, F::PRO_AFTER>(nxt, Et, chit, st, V, toff + (m + 1) * mstep);
+                if (m + 1 < E) epi_load<T, NC, CK, S_INLINE>(nxt, Et, chit, st, V, toff + (m + 1) * mstep);
                 const uint32_t bo = toff + m * mstep;
                 cx<T> w[3] = {zero, zero, zero}, d[3] = {zero, zero, zero}, en[3] = {zero, zero, zero};
 #pragma unroll
@@ -246,9 +263,25 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
                     cx<T> x[3] = {zero, zero, zero};
                     chi_mul_ck<T, CK, NC>(cur.q, en, x, cf);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) v[c][m] = x[c] + scal(cur.s[c], cf.a);
+                    for (int c = 0; c < NC; ++c) {
+                        if constexpr (S_INLINE) v[c][m] = x[c] + scal(cur.s[c], cf.a);
+                        else v[c][m] = x[c];
+                    }
                 }
                 if (m + 1 < E) cur = nxt;
+            }
+            if constexpr (F::PRO_AFTER && SB != 0) {
+                if (s_row) {
+#pragma unroll
+                    for (int m = 0; m < E; ++m) {
+                        const int j = t + m * TP;
+                        if (j >= s_jlo && j < s_jhi) {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c)
+                                v[c][m] = v[c][m] + scal(at(st + c * V, toff + m * mstep), cf.a);
+                        }
+                    }
+                }
             }
         }
         write_partials(dd, ee, a, blk, red);

@@ -341,6 +341,7 @@ class Solution:
         S = S.to(torch.complex128).expand(*full)
         self.__source_normalized = (S * (1.0j / self.__alpha_base.imag / self.__beta)).to(
             BE.hardware_dtype).contiguous()
+        self.__source_box = source_support(self.__source_normalized, self.grid.ndim)
         self.__previous_update_norm = math.inf
         self.__work_valid = False
 
@@ -470,10 +471,10 @@ class Solution:
         BE = self.__BE
         if not self.__work_valid:
             BE.born_prologue(self.__field_array, self.__work, self.__source_normalized, self.__chi, self.__chi_kind,
-                             self.__coef)
+                             self.__coef, s_box=self.__source_box)
         # completes this iteration and, speculating that it is accepted, computes the next one's first pass
         BE.born_iterate(self.__field_array, self.__field_next, self.__work, self.__source_normalized, self.__chi,
-                        self.__chi_kind, self.__coef, self.__norms)
+                        self.__chi_kind, self.__coef, self.__norms, s_box=self.__source_box)
         self.__work_valid = True
         d2, e2 = self.__norms.tolist()  # the one host sync of the iteration
         current = math.sqrt(d2)
@@ -495,6 +496,7 @@ class Solution:
         """Device operands for a batched solve: chi and s for the set-up alpha, the affine coefficients, the
         current field and the iteration bookkeeping."""
         return {'chi': self.__chi, 'chi_kind': self.__chi_kind, 's': self.__source_normalized,
+                's_box': self.__source_box,
                 'coef': self.__coef, 'field': self.__field_array, 'alpha': self.__alpha, 'alpha_base': self.__alpha_base,
                 'beta': self.__beta, 'coef_a': self.__coef_a, 'coef_c': self.__coef_c,
                 'prev': self.__previous_update_norm, 'iteration': self.__iteration,
@@ -543,6 +545,23 @@ class Solution:
             if not callback(sol):
                 break
         return self
+
+
+def source_support(s: torch.Tensor, ndim: int):
+    """Bounding box (lo, hi) of the nonzero voxels of a [..., *grid] source tensor (empty box: lo == hi)."""
+    nz = s != 0
+    while nz.ndim > ndim:
+        nz = nz.any(dim=0)
+    lo, hi = [], []
+    for a in range(ndim):
+        other = tuple(d for d in range(ndim) if d != a)
+        line = nz.any(dim=other) if other else nz
+        idx = torch.nonzero(line).ravel()
+        if idx.numel() == 0:
+            return [0] * ndim, [0] * ndim
+        lo.append(int(idx[0]))
+        hi.append(int(idx[-1]) + 1)
+    return lo, hi
 
 
 def _eigvals(m: torch.Tensor) -> torch.Tensor:
