@@ -225,14 +225,24 @@ class BatchSolution:
     def all_done(self) -> bool:
         return bool(((self.__state & _native.STATE_DONE) != 0).all())
 
+    def tune(self, **kwargs) -> dict:
+        """Autotune the batch's plan for this batch (``macromax_amd.autotune.tune_plan``)."""
+        from .autotune import tune_plan
+        return tune_plan(self.__plan, self.__ncomp, self.__chi_kind, nbatch=self.__nbatch,
+                         chi_shared=self.__chi_bstride == 0, **kwargs)
+
     def solve(self, tol: float = 1e-4, max_iter: int = 10000, steps_per_sync: int = 16, graph: bool = True,
-              write_back: bool = True) -> 'BatchSolution':
+              write_back: bool = True, tune: bool = False) -> 'BatchSolution':
         """Iterate every problem until ``not (residue > tol and iteration < max_iter)``.
 
         :param steps_per_sync: iterations launched (or captured in one graph) between host checks.
         :param graph: capture ``steps_per_sync`` iterations in a graph once and replay it.
         :param write_back: hand each problem's final state back to its ``Solution`` (batches built from Solutions).
+        :param tune: autotune the plan's tile shapes for this batch first (cached per size).
         """
+        if tune:
+            self.tune()
+            self.__graph = None
         if graph:
             key = (float(tol), int(max_iter), int(steps_per_sync))
             if self.__graph is None or self.__graph_key != key:
@@ -282,9 +292,10 @@ def _stamp_all(batches):
 
 
 def solve_many(solutions: Sequence[Solution], tol: float = 1e-4, max_iter: int = 10000, steps_per_sync: int = 16,
-               concurrent: bool = True, graph: bool = True):
+               concurrent: bool = True, graph: bool = True, tune: bool = False):
     """Solve problems of uneven grid sizes: one ``BatchSolution`` per grid shape (and element type / chi storage),
-    the groups' iterations on separate streams of one captured graph when ``concurrent``.
+    the groups' iterations on separate streams of one captured graph when ``concurrent``; ``tune`` autotunes every
+    group's plan for its batch first.
 
     Each ``Solution`` receives its final state. Returns the list of batches (their ``latency()`` gives the
     per-problem stop times from the common start).
@@ -295,6 +306,9 @@ def solve_many(solutions: Sequence[Solution], tol: float = 1e-4, max_iter: int =
     batches = [BatchSolution(g) for g in groups.values()]
     if not batches:
         return []
+    if tune:
+        for bt in batches:
+            bt.tune()
     dev = batches[0].field_arrays().device
     if graph:
         for bt in batches:
