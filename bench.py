@@ -55,21 +55,22 @@ def build_anisotropic_medium(n, device, seed=2, n_seeds=2000, no=1.50, ne=1.65, 
     return eps
 
 
-def setup_solution(n, device):
+def setup_solution(n, device, dtype=np.complex64):
     import torch
     import macromax_amd as mm
     from macromax_amd.bound import LinearBound
     wavelength = 1e-6
     pitch = 250e-9
     grid = mm.Grid(shape=[n, n, n], step=pitch)
-    eps = build_anisotropic_medium(n, device)
-    J = torch.zeros((3, n, n, n), dtype=torch.complex64, device=device)
+    tdt = torch.complex64 if dtype == np.complex64 else torch.complex128
+    eps = build_anisotropic_medium(n, device).to(tdt)
+    J = torch.zeros((3, n, n, n), dtype=tdt, device=device)
     J[0, :, :, 20] = 1.0 / math.sqrt(2)
     J[1, :, :, 20] = 1.0j / math.sqrt(2)
     bound = LinearBound(grid, thickness=[[0, 0], [0, 0], [16 * pitch, 16 * pitch]])
     t0 = time.perf_counter()
     sol = mm.Solution(grid, vacuum_wavelength=wavelength, epsilon=eps, current_density=J, bound=bound,
-                      dtype=np.complex64)
+                      dtype=dtype)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     del eps, J
@@ -165,6 +166,16 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
         seq_t.append(dt)
     log(f'part A sequential: {sum(seq_t):.2f} s')
     bat = [disc_problem(int(n), 1000 + i) for i, n in enumerate(sizes)]
+    # autotune every size group for its batch (timed separately; cached per size)
+    from macromax_amd.batch import BatchSolution as _BS, _group_key
+    groups = {}
+    for sol_ in bat:
+        groups.setdefault(_group_key(sol_), []).append(sol_)
+    tune_s = 0.0
+    for g_ in groups.values():
+        _, dt = _timed(lambda: _BS(g_).tune())
+        tune_s += dt
+    log(f'part A autotune: {tune_s:.2f} s')
     batches, bt_total = _timed(lambda: solve_many(bat, tol=tol, max_iter=max_iter, steps_per_sync=steps_per_sync))
     log(f'part A batched: {bt_total:.2f} s')
     lat = np.concatenate([b.latency() for b in batches])
@@ -178,6 +189,7 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
         'sequential_latency_p50_s': round(float(np.percentile(seq_t, 50)), 4),
         'sequential_latency_p99_s': round(float(np.percentile(seq_t, 99)), 4),
         'iterations_median': int(np.median(its)), 'iterations_max': int(its.max()),
+        'autotune_s': round(tune_s, 2),
         'iterations_equal_to_sequential': bool(all(a.iteration == b.iteration for a, b in zip(bat, seq))),
         'max_rel_l2_vs_sequential': float(max(errs)),
         'timed': 'solve_many end to end (batch set-up, graph capture, replays, write-back); sequential: '
@@ -188,6 +200,8 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
     n = 512
     src = plane_wave_sources(n, nrhs)
     medium = disc_problem(n, 0)
+    bt = BatchSolution.from_sources(medium, current_densities=src)
+    _, tune_b = _timed(lambda: bt.tune())
     bt, bt_total = _timed(lambda: BatchSolution.from_sources(medium, current_densities=src).solve(
         tol=tol, max_iter=max_iter, steps_per_sync=steps_per_sync))
     log(f'part B batched: {bt_total:.2f} s')
@@ -210,7 +224,7 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
         'latency_p50_s': round(float(np.percentile(lat, 50)), 4), 'latency_p99_s': round(float(np.percentile(lat, 99)), 4),
         'sequential_latency_p50_s': round(float(np.percentile(seq_t, 50)), 4),
         'sequential_latency_p99_s': round(float(np.percentile(seq_t, 99)), 4),
-        'iterations_median': int(np.median(its_b)), 'iterations_max': int(its_b.max()),
+        'iterations_median': int(np.median(its_b)), 'iterations_max': int(its_b.max()), 'autotune_s': round(tune_b, 2),
         'iterations_equal_to_sequential': bool(all(its_eq)), 'max_rel_l2_vs_sequential': float(max(errs)),
         'timed': 'BatchSolution.from_sources(...).solve end to end; sequential: Solution.solve per right-hand side'}
     return out
@@ -318,6 +332,8 @@ def main():
     ap.add_argument('--residual', type=float, default=1e-5)
     ap.add_argument("--max-iter", type=int, default=10000)
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-solve comparison')
+    ap.add_argument('--no-tune', action='store_true', help='keep the library default tile shapes (no autotuning)')
+    ap.add_argument('--no-accuracy', action='store_true', help='skip the complex128 reference solve')
     ap.add_argument('--batched-only', action='store_true', help='only the batched-solve comparison (own JSON line)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient measurement')
     ap.add_argument('--adjoint-only', action='store_true', help='only the adjoint-gradient measurement')
@@ -342,6 +358,11 @@ def main():
     print(f'bench: building the {n}^3 anisotropic workload', file=sys.stderr, flush=True)
     sol, setup_s = setup_solution(n, dev)
     print(f'bench: set-up {setup_s:.1f} s, alpha = {sol.alpha}', file=sys.stderr, flush=True)
+    tuned = None
+    if not args.no_tune:
+        from macromax_amd.autotune import tune_solution
+        tuned = tune_solution(sol)
+        log(f'autotune: {tuned}')
 
     # time-to-residual from E = 0 (includes the per-iteration host read of the two norms)
     torch.cuda.synchronize()
@@ -403,6 +424,26 @@ def main():
         'time_to_residual_s': round(ttr, 4), 'iterations_to_residual': iters_to_res, 'residual': residue,
         'setup_s': round(setup_s, 2), 'alpha': [sol.alpha.real, sol.alpha.imag],
     }
+    if tuned is not None:
+        out['autotune'] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()}
+    if not args.no_accuracy:
+        # the same problem in complex128 (the double-precision kernels, pinned to the oracle at 1e-11 per iteration
+        # on small grids) solved further, as the reference for the complex64 field at residual 1e-5
+        E64 = sol.field_array.clone()
+        ref, _ = setup_solution(n, dev, dtype=np.complex128)
+        t0 = time.perf_counter()
+        ref.solve(callback=lambda s_: s_.residue > 1e-7 and s_.iteration < 3 * args.max_iter)
+        torch.cuda.synchronize()
+        t128 = time.perf_counter() - t0
+        Eref = ref.field_array
+        rel = float(torch.linalg.vector_norm(E64.to(torch.complex128) - Eref) / torch.linalg.vector_norm(Eref))
+        out['accuracy'] = {'rel_l2_c64_vs_c128': rel, 'c64_residual': residue, 'c128_residual': ref.residue,
+                           'c128_iterations': ref.iteration, 'c128_solve_s': round(t128, 2),
+                           'note': 'complex64 field at residual 1e-5 against the complex128 solve of the same '
+                                   'problem at residual 1e-7'}
+        log(f'accuracy: c64 vs c128 relative L2 {rel:.2e} ({ref.iteration} c128 iterations, {t128:.1f} s)')
+        del ref, Eref, E64
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         print('bench: CPU baseline (one complex128 oracle iteration)', file=sys.stderr, flush=True)
         cb = cpu_baseline(sol, n)

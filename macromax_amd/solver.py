@@ -110,6 +110,8 @@ class Solution:
         self.__work = BE.allocate_array(fill_value=0)  # first pass of the next iteration (FFT along axis -1)
         self.__work_valid = False
         self.__norms = torch.zeros(2, dtype=torch.float64, device=BE.device)
+        self.__field_fp64 = None
+        self.__BE64 = None
         self.E = initial_field
 
         mu = func2arr(mu)
@@ -398,6 +400,7 @@ class Solution:
         if t.ndim == self.grid.ndim + 1:  # [C, *grid] -> [C, 1, *grid]
             t = t.unsqueeze(1)
         BE.assign(t, self.__field_array)
+        self.__field_fp64 = None
         self.__previous_update_norm = math.inf
         self.__field_norm = None
         self.__work_valid = False
@@ -482,6 +485,7 @@ class Solution:
             rel = np.float64(current) / np.float64(self.__previous_update_norm)
         if rel < 1:
             self.__field_array, self.__field_next = self.__field_next, self.__field_array
+            self.__field_fp64 = None
             self.__previous_update_norm = current
             self.__field_norm = math.sqrt(e2)
             return True
@@ -490,6 +494,79 @@ class Solution:
         log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
         self.__update_operators(alpha_new)  # also marks the speculative first pass in the work buffer stale
         return False
+
+    # ------------------------------------------------------------------ FP64 refinement
+    def __fp64_backend(self):
+        if self.__BE64 is None:
+            from .backend.hip import BackEndHip
+            self.__BE64 = BackEndHip(1 + 2 * self.vectorial, self.grid * self.wavenumber, np.complex128,
+                                     device=self.__BE.device)
+        return self.__BE64
+
+    def __fp64_residual_source(self, E: torch.Tensor):
+        """s_c = s + chi E - G^-1 E in FP64, for the current alpha: the Born update of E is chi G s_c and the error
+        of E solves the iteration with source s_c."""
+        BE64 = self.__fp64_backend()
+        chi = self.chi_effective.to(torch.complex128)
+        s = self.source_normalized.to(torch.complex128)
+        chiE = chi * E if chi.shape[0] == 1 else torch.einsum('ij...,jk...->ik...', chi, E)
+        a = self.__alpha
+        F = BE64.ft(E)
+        ginv = (BE64.k2.to(torch.complex128) - a) * (1.0j / a.imag)  # 1/g, g = -i alpha_i / (k^2 - alpha)
+        if self.vectorial:
+            PL = BE64.longitudinal_projection_ft(F)
+            F = ginv * (F - PL) + PL * (-1.0j * a / a.imag)  # 1 / (i alpha_i / alpha) on the longitudinal part
+        else:
+            F = ginv * F
+        return s + chiE - BE64.ift(F), chi
+
+    def __fp64_update_norm(self, s_c: torch.Tensor, chi: torch.Tensor) -> float:
+        BE64 = self.__fp64_backend()
+        a = self.__alpha
+        F = BE64.ft(s_c)
+        g = (-1.0j * a.imag) / (BE64.k2.to(torch.complex128) - a)
+        if self.vectorial:
+            PL = BE64.longitudinal_projection_ft(F)
+            F = g * (F - PL) + PL * (1.0j * a.imag / a)
+        else:
+            F = g * F
+        G = BE64.ift(F)
+        d = chi * G if chi.shape[0] == 1 else torch.einsum('ij...,jk...->ik...', chi, G)
+        return float(torch.linalg.vector_norm(d))
+
+    @property
+    def field_array_fp64(self) -> torch.Tensor:
+        """The field in double precision: the refined field after ``refine()``, else the field converted."""
+        f = self.__field_fp64
+        return f if f is not None else self.__field_array.to(torch.complex128)
+
+    def refine(self, tol: float = 3e-5, max_iter: int = 10000, steps_per_sync: int = 16) -> dict:
+        """One mixed-precision refinement sweep (no reference counterpart; the reference iterates in one precision).
+
+        The Born map is affine, so the error e = E* - E of the current field solves the same iteration with the
+        source s_c = s + chi E - G^-1 E (evaluated in FP64 with the double-precision kernels); e is solved with the
+        complex64 kernels to relative residue ``tol`` and added in FP64. One sweep multiplies the error by about the
+        relative accuracy of the correction solve, so a complex64 solution is brought below the single-precision
+        floor of the iteration itself. Returns the FP64 relative residues before and after and the iterations used.
+        """
+        from .batch import BatchSolution
+        E = self.field_array_fp64.clone()
+        s_c, chi = self.__fp64_residual_source(E)
+        res_before = self.__fp64_update_norm(s_c, chi) / float(torch.linalg.vector_norm(E))
+        # source distribution S_c whose normalised, coefficient-scaled form is s_c
+        S_c = s_c * (self.__alpha_base.imag * self.__beta / (1.0j * self.__coef_a))
+        bt = BatchSolution.from_sources(self, sources=S_c[None]).solve(tol=tol, max_iter=max_iter,
+                                                                      steps_per_sync=steps_per_sync, graph=False)
+        E = E + bt.field_arrays()[0].to(torch.complex128)
+        self.__field_fp64 = E
+        self.__field_array.copy_(E.to(self.__field_array.dtype))
+        self.__work_valid = False
+        self.__field_norm = None
+        self.__relative_residue = None
+        s_c, chi = self.__fp64_residual_source(E)
+        res_after = self.__fp64_update_norm(s_c, chi) / float(torch.linalg.vector_norm(E))
+        return {'fp64_residue_before': res_before, 'fp64_residue_after': res_after,
+                'correction_iterations': int(bt.iteration[0]), 'correction_residue': float(bt.residue[0])}
 
     # ------------------------------------------------------------------ batched solves (macromax_amd.batch)
     def _batch_operands(self) -> dict:
@@ -521,6 +598,7 @@ class Solution:
                       coef_a: float, coef_c: complex):
         """Take over the state a batched solve reached for this problem."""
         self.__field_array.copy_(field)
+        self.__field_fp64 = None
         self.__iteration = int(iteration)
         self.__previous_update_norm = float(prev)
         self.__field_norm = float(field_norm) if field_norm > 0 else None
