@@ -311,6 +311,44 @@ def adjoint_benchmark(n=96, design=48, tol=1e-6, max_iter=10000, n_fd=50, h=1e-3
                  'gradient assembly (loss.backward()); forward_solve_only = Solution.solve without set-up'}
 
 
+def accuracy_benchmark(n, dev, E_plain, plain_residue, log):
+    """The complex64 field against a complex128 solve of the same problem converged to residual 1e-10, at residual
+    1e-5 (the plain solve) and after FP64 refinement: a complex64 solve to residual 1e-4 and two refinement sweeps
+    (correction solves to 1e-4), which is timed as time-to-accuracy."""
+    import torch
+    ref, _ = setup_solution(n, dev, dtype=np.complex128)
+    t0 = time.perf_counter()
+    ref.solve(callback=lambda s_: s_.residue > 1e-10 and s_.iteration < 60000)
+    torch.cuda.synchronize()
+    t128 = time.perf_counter() - t0
+    Eref = ref.field_array.clone()
+    nref = torch.linalg.vector_norm(Eref)
+    out = {'reference': {'dtype': 'c128', 'residual': ref.residue, 'iterations': ref.iteration,
+                         'solve_s': round(t128, 2)}}
+    del ref
+    torch.cuda.empty_cache()
+    err = lambda E: float(torch.linalg.vector_norm(E.to(torch.complex128) - Eref) / nref)
+    out['plain_c64'] = {'residual': plain_residue, 'rel_l2_vs_c128': err(E_plain)}
+    sol, _ = setup_solution(n, dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol.solve(callback=lambda s_: s_.residue > 1e-4 and s_.iteration < 20000)
+    its = sol.iteration
+    sweeps = []
+    for _ in range(2):
+        info = sol.refine(tol=1e-4)
+        its += info['correction_iterations']
+        sweeps.append({k: (float(v) if isinstance(v, float) else v) for k, v in info.items()})
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    out['refined_c64'] = {'schedule': 'solve to residual 1e-4, then 2 FP64 refinement sweeps (correction solves to '
+                                      '1e-4)', 'rel_l2_vs_c128': err(sol.field_array_fp64), 'time_s': round(t_ref, 2),
+                          'iterations': int(its), 'sweeps': sweeps}
+    log(f"accuracy: plain c64 at 1e-5 {out['plain_c64']['rel_l2_vs_c128']:.2e}; refined "
+        f"{out['refined_c64']['rel_l2_vs_c128']:.2e} in {t_ref:.1f} s; c128 reference {t128:.1f} s")
+    return out
+
+
 def measured_traffic(n):
     """HBM bytes per voxel-iteration from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
     path = os.path.join(ROOT, 'profiles', 'born_iterate_256cubed_aniso.md')
@@ -411,6 +449,9 @@ def main():
     if traffic is not None:
         roofline['traffic_bytes_per_voxel_iter'] = traffic
         roofline['actual_GBps'] = round(traffic * V / (gpu_ms_per_iter * 1e-3) / 1e9, 1)
+        # the fused schedule moves fewer bytes than the algorithmic count, so frac (algorithmic) can exceed 1;
+        # frac_actual is the measured traffic's share of the peak
+        roofline['frac_actual'] = round(roofline['actual_GBps'] / HBM_PEAK_GBS, 4)
 
     out = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'voxel-iter/s', 'n_gpus': 1, 'steps': args.steps,
@@ -427,22 +468,7 @@ def main():
     if tuned is not None:
         out['autotune'] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()}
     if not args.no_accuracy:
-        # the same problem in complex128 (the double-precision kernels, pinned to the oracle at 1e-11 per iteration
-        # on small grids) solved further, as the reference for the complex64 field at residual 1e-5
-        E64 = sol.field_array.clone()
-        ref, _ = setup_solution(n, dev, dtype=np.complex128)
-        t0 = time.perf_counter()
-        ref.solve(callback=lambda s_: s_.residue > 1e-7 and s_.iteration < 3 * args.max_iter)
-        torch.cuda.synchronize()
-        t128 = time.perf_counter() - t0
-        Eref = ref.field_array
-        rel = float(torch.linalg.vector_norm(E64.to(torch.complex128) - Eref) / torch.linalg.vector_norm(Eref))
-        out['accuracy'] = {'rel_l2_c64_vs_c128': rel, 'c64_residual': residue, 'c128_residual': ref.residue,
-                           'c128_iterations': ref.iteration, 'c128_solve_s': round(t128, 2),
-                           'note': 'complex64 field at residual 1e-5 against the complex128 solve of the same '
-                                   'problem at residual 1e-7'}
-        log(f'accuracy: c64 vs c128 relative L2 {rel:.2e} ({ref.iteration} c128 iterations, {t128:.1f} s)')
-        del ref, Eref, E64
+        out['accuracy'] = accuracy_benchmark(n, dev, sol.field_array.clone(), residue, log)
         torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         print('bench: CPU baseline (one complex128 oracle iteration)', file=sys.stderr, flush=True)
