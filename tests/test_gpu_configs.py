@@ -1,0 +1,92 @@
+"""Parity at the sizes of the benchmark configurations (the workloads of bench.py).
+
+- 256^2 scalar complex128 disc medium (2D Helmholtz, 32-pixel LinearBound, Gaussian line source): the GPU solve to
+  residue 1e-6 against the complex128 oracle solved with the same alpha and stopping rule: same iteration count,
+  fields within 1e-9 relative L2;
+- 128^3 isotropic vectorial complex64 (Gaussian-smoothed random index in [1.0, 1.6], 16-pixel LinearBound on all
+  faces, x-polarised Gaussian current on z = 20): the first 8 iterates against the complex128 oracle, <= 1e-5
+  relative L2 each;
+- 256^3 anisotropic vectorial complex64 (the headline workload): the first 2 iterates against the oracle, <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from oracle import born_oracle as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(sol):
+    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    s = sol.source_normalized.cpu().numpy().astype(np.complex128)
+    k = ob.k_vectors(tuple(sol.grid.shape), np.asarray(sol.grid.step) * sol.wavenumber)
+    return chi, s, k
+
+
+def _rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+def test_disc_medium_256_scalar_complex128_full_solve():
+    import bench
+    print('256^2 scalar c128 full solve', flush=True)
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    n, pitch = 256, 125e-9
+    grid = mm.Grid(shape=[n, n], step=pitch)
+    idx = bench.disc_medium(n, 0, pitch=pitch)
+    y = (np.arange(n) - n / 2) * pitch
+    J = np.zeros((1, n, n), dtype=np.complex128)
+    J[0, 40, :] = np.exp(-(y / 3e-6) ** 2)
+    sol = mm.Solution(grid, vacuum_wavelength=500e-9, epsilon=idx ** 2, current_density=J,
+                      bound=LinearBound(grid, thickness=32 * pitch, max_extinction_coefficient=0.2),
+                      dtype=np.complex128)
+    chi, s, k = _oracle(sol)
+    o = ob.solve_to(chi, s, sol.alpha, sol.beta, k, False, tol=1e-6)
+    sol.solve(lambda s_: s_.residue > 1e-6 and s_.iteration < 10000)
+    assert sol.iteration == o.iteration
+    assert _rel(sol.field_array.cpu().numpy(), o.E) < 1e-9
+    np.testing.assert_allclose(sol.residue, o.relative_residue, rtol=1e-8)
+
+
+def _smooth_index(n, seed=1, sigma=2.0):
+    import scipy.ndimage
+    noise = np.random.default_rng(seed).standard_normal((n, n, n))
+    sm = scipy.ndimage.gaussian_filter(noise, sigma, mode='wrap')
+    sm = (sm - sm.min()) / (sm.max() - sm.min())
+    return 1.0 + 0.6 * sm
+
+
+def test_smooth_medium_128_vectorial_complex64_trajectory():
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print('128^3 iso vectorial c64 trajectory', flush=True)
+    n, pitch = 128, 250e-9
+    grid = mm.Grid(shape=[n, n, n], step=pitch)
+    idx = _smooth_index(n)
+    x = (np.arange(n) - n / 2) * pitch
+    J = np.zeros((3, n, n, n), dtype=np.complex64)
+    J[0, :, :, 20] = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2e-6) ** 2)
+    sol = mm.Solution(grid, vacuum_wavelength=1e-6, epsilon=idx ** 2, current_density=J,
+                      bound=LinearBound(grid, thickness=16 * pitch), dtype=np.complex64)
+    chi, s, k = _oracle(sol)
+    o = ob.BornOracle(chi, s, sol.alpha, sol.beta, k, True)
+    for it in range(1, 9):
+        sol.step()
+        o.step()
+        err = _rel(sol.field_array.cpu().numpy(), o.E)
+        assert err < 1e-5, (it, err)
+
+
+def test_anisotropic_256_complex64_first_iterates():
+    import bench
+    import torch
+    print('256^3 aniso c64 first iterates', flush=True)
+    sol, _ = bench.setup_solution(256, torch.device('cuda:0'))
+    chi, s, k = _oracle(sol)
+    o = ob.BornOracle(chi, s, sol.alpha, sol.beta, k, True)
+    for it in range(1, 3):
+        sol.step()
+        o.step()
+        err = _rel(sol.field_array.cpu().numpy(), o.E)
+        assert err < 1e-5, (it, err)
