@@ -89,17 +89,18 @@ def _candidates(plan):
     return out
 
 
-def _time_iterate(plan, bufs, ncomp, chi_kind, nbatch, chi_bstride, iterations, repeats):
+def _time_iterate(plan, bufs, ncomp, chi_kind, nbatch, chi_bstride, s_bstride, s_box, iterations, repeats):
     E, Eo, W, s, chi, coef, norms = bufs
-    plan.born_prologue(E, W, s, chi, ncomp, chi_kind, coef, nbatch, ncomp * plan.V, chi_bstride)
-    plan.born_iterate(E, Eo, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, ncomp * plan.V, chi_bstride)
+    plan.born_prologue(E, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, s_box=s_box)
+    plan.born_iterate(E, Eo, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride, chi_bstride, s_box=s_box)
     times = []
     for _ in range(repeats):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(iterations):
-            plan.born_iterate(E, Eo, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, ncomp * plan.V, chi_bstride)
+            plan.born_iterate(E, Eo, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride, chi_bstride,
+                              s_box=s_box)
         e1.record()
         e1.synchronize()
         times.append(e0.elapsed_time(e1) / iterations)
@@ -107,9 +108,12 @@ def _time_iterate(plan, bufs, ncomp, chi_kind, nbatch, chi_bstride, iterations, 
 
 
 def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool = True, iterations: int = 5,
-              repeats: int = 3, use_cache: bool = True, verbose: bool = False) -> dict:
+              repeats: int = 5, use_cache: bool = True, verbose: bool = False, operands=None) -> dict:
     """Tune ``plan`` for ``nbatch`` problems of ``ncomp`` components and chi kind ``chi_kind``; the best parameters
-    are set on the plan and returned (with the measured milliseconds per iteration under ``'ms'``)."""
+    are set on the plan and returned (with the measured milliseconds per iteration under ``'ms'``).
+
+    ``operands`` = (s, chi, coef, s_box) of a real problem (one problem, ``nbatch`` 1) times the kernels that problem
+    uses (e.g. the source-box variant); otherwise random dense operands of the right sizes are used."""
     key = cache_key(plan.shape, plan.dtype, ncomp, chi_kind, nbatch, chi_shared)
     if use_cache:
         with _lock:
@@ -126,10 +130,18 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
                              torch.randn(*shape, device=dev, generator=g)).to(plan.dtype) * 0.1
     nchi = {1: 1, 2: 9, 3: 6}[chi_kind]
     E = crand(nbatch, ncomp, V)
-    bufs = (E, torch.empty_like(E), torch.empty_like(E), crand(nbatch, ncomp, V),
-            crand(1 if chi_shared else nbatch, nchi, V),
-            torch.tensor([1.2, 0.4, 1.0, 0.0, 0.0, 0.0] * nbatch, dtype=plan.real_dtype, device=dev),
-            torch.zeros(2 * nbatch, dtype=torch.float64, device=dev))
+    if operands is not None:
+        s_op, chi_op, coef_op, s_box = operands
+        bufs = (E, torch.empty_like(E), torch.empty_like(E), s_op, chi_op, coef_op,
+                torch.zeros(2 * nbatch, dtype=torch.float64, device=dev))
+        s_bstride = 0
+    else:
+        s_box = None
+        bufs = (E, torch.empty_like(E), torch.empty_like(E), crand(nbatch, ncomp, V),
+                crand(1 if chi_shared else nbatch, nchi, V),
+                torch.tensor([1.2, 0.4, 1.0, 0.0, 0.0, 0.0] * nbatch, dtype=plan.real_dtype, device=dev),
+                torch.zeros(2 * nbatch, dtype=torch.float64, device=dev))
+        s_bstride = ncomp * V
     chi_bstride = 0 if chi_shared else nchi * V
     default_xcd = plan.get_param('xcd_remap')
     defaults = {name: (default_xcd if values[0] is None else values[0]) for name, values in PARAMETERS.items()}
@@ -140,7 +152,8 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
     def measure(params):
         apply(plan, defaults)
         apply(plan, params)
-        return _time_iterate(plan, bufs, ncomp, chi_kind, nbatch, chi_bstride, iterations, repeats)
+        return _time_iterate(plan, bufs, ncomp, chi_kind, nbatch, chi_bstride, s_bstride, s_box, iterations,
+                             repeats)
 
     best = {name: values[0] for name, values in cands.items()}
     best_ms = measure(best)
@@ -153,7 +166,7 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
             ms = measure(trial)
             if verbose:
                 print(f'autotune {plan.shape} {name}={v}: {ms:.4f} ms', flush=True)
-            if ms < best_ms * 0.98:  # require a clear gain over the current choice
+            if ms < best_ms * 0.97:  # require a clear gain over the current choice (timing noise is ~1-2 %)
                 best, best_ms = trial, ms
     measure(best)  # leave the plan configured with the winner
     result = dict(defaults)
@@ -166,7 +179,8 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
     return dict(result)
 
 
-def tune_solution(solution, nbatch: int = 1, **kwargs) -> dict:
-    """Tune the plan of a ``Solution``'s backend for its own problem (``nbatch`` copies of it)."""
+def tune_solution(solution, **kwargs) -> dict:
+    """Tune the plan of a ``Solution``'s backend on its own chi, source (and source box) and coefficients."""
     o = solution._batch_operands()
-    return tune_plan(solution.backend.plan, o['ncomp'], o['chi_kind'], nbatch=nbatch, **kwargs)
+    return tune_plan(solution.backend.plan, o['ncomp'], o['chi_kind'], nbatch=1,
+                     operands=(o['s'], o['chi'], o['coef'], o['s_box']), **kwargs)

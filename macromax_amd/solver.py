@@ -105,11 +105,15 @@ class Solution:
         # the backend works in units of 1/k0 (solver.py:212)
         self.__BE = backend.load(1 + 2 * self.vectorial, self.grid * self.wavenumber, dtype=dtype)
         BE = self.__BE
-        self.__field_array = BE.allocate_array(fill_value=0)
-        self.__field_next = BE.allocate_array(fill_value=0)
+        # three field buffers: the current field, the output of the iteration in flight and that of the next one,
+        # launched before the current one is decided (discarded if the current update is rejected)
+        self.__fields = [BE.allocate_array(fill_value=0) for _ in range(3)]
+        self.__cur = 0
+        self.__pending = None
         self.__work = BE.allocate_array(fill_value=0)  # first pass of the next iteration (FFT along axis -1)
         self.__work_valid = False
-        self.__norms = torch.zeros(2, dtype=torch.float64, device=BE.device)
+        self.__norms_dev = [torch.zeros(2, dtype=torch.float64, device=BE.device) for _ in range(2)]
+        self.__norms_host = [torch.zeros(2, dtype=torch.float64, pin_memory=True) for _ in range(2)]
         self.__field_fp64 = None
         self.__BE64 = None
         self.E = initial_field
@@ -218,7 +222,7 @@ class Solution:
         self.source_distribution = source_distribution
 
     def __upload_coef(self):
-        self.__work_valid = False
+        self.__invalidate_work()
         a = self.__alpha
         vals = [a.real, a.imag, self.__coef_a, self.__coef_c.real, self.__coef_c.imag, 0.0]
         self.__coef.copy_(torch.tensor(vals, dtype=torch.float64))
@@ -345,7 +349,7 @@ class Solution:
             BE.hardware_dtype).contiguous()
         self.__source_box = source_support(self.__source_normalized, self.grid.ndim)
         self.__previous_update_norm = math.inf
-        self.__work_valid = False
+        self.__invalidate_work()
 
     @property
     def j(self) -> np.ndarray:
@@ -390,7 +394,7 @@ class Solution:
     @property
     def E(self) -> np.ndarray:
         """Electric field in SI units, [C, *grid] (``solver.py:811-821``)."""
-        return self.__BE.asnumpy(self.__field_array[:, 0] / (self.wavenumber ** 2))
+        return self.__BE.asnumpy(self.__fields[self.__cur][:, 0] / (self.wavenumber ** 2))
 
     @E.setter
     def E(self, E):
@@ -399,16 +403,16 @@ class Solution:
         t = BE.astype(val) * (self.wavenumber ** 2)
         if t.ndim == self.grid.ndim + 1:  # [C, *grid] -> [C, 1, *grid]
             t = t.unsqueeze(1)
-        BE.assign(t, self.__field_array)
+        BE.assign(t, self.__fields[self.__cur])
         self.__field_fp64 = None
         self.__previous_update_norm = math.inf
         self.__field_norm = None
-        self.__work_valid = False
+        self.__invalidate_work()
 
     @property
     def field_array(self) -> torch.Tensor:
         """The k0^2-scaled field on the device, [C, 1, *grid] (read-only view of the current buffer)."""
-        return self.__field_array
+        return self.__fields[self.__cur]
 
     @property
     def B(self) -> np.ndarray:
@@ -456,7 +460,7 @@ class Solution:
         if self.__relative_residue is None:
             if self.__previous_update_norm > 0:
                 if self.__field_norm is None:
-                    self.__field_norm = self.__BE.norm(self.__field_array)
+                    self.__field_norm = self.__BE.norm(self.__fields[self.__cur])
                 self.__relative_residue = self.__previous_update_norm / self.__field_norm
             else:
                 self.__relative_residue = 0.0
@@ -467,24 +471,50 @@ class Solution:
         return self.relative_residue
 
     # ------------------------------------------------------------------ iteration (solver.py:1049-1098)
+    def __invalidate_work(self):
+        """The work buffer no longer holds the first pass of the current field, and an iteration launched ahead
+        is void (the field, the source, the medium or alpha changed)."""
+        self.__work_valid = False
+        self.__pending = None
+
+    def __launch_iteration(self, src: int, dst: int, k: int):
+        """Enqueue one fused iteration from field buffer src into dst, its norms into slot k (copied to pinned host
+        memory behind an event). Returns (dst, k, event)."""
+        BE = self.__BE
+        BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
+                        self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box)
+        self.__norms_host[k].copy_(self.__norms_dev[k], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(BE.device))
+        return dst, k, ev
+
     def step(self) -> bool:
-        """One Born iteration; returns True if the update was accepted."""
+        """One Born iteration; returns True if the update was accepted.
+
+        The next iteration is enqueued before this one's norms are read (assuming acceptance, which holds for
+        nearly every iteration), so the device never waits for the host's decision; it is discarded when the update
+        is rejected."""
         self.__iteration += 1
         self.__relative_residue = None
         BE = self.__BE
-        if not self.__work_valid:
-            BE.born_prologue(self.__field_array, self.__work, self.__source_normalized, self.__chi, self.__chi_kind,
-                             self.__coef, s_box=self.__source_box)
-        # completes this iteration and, speculating that it is accepted, computes the next one's first pass
-        BE.born_iterate(self.__field_array, self.__field_next, self.__work, self.__source_normalized, self.__chi,
-                        self.__chi_kind, self.__coef, self.__norms, s_box=self.__source_box)
+        cur = self.__cur
+        if self.__pending is not None:
+            out, k, ev = self.__pending
+        else:
+            if not self.__work_valid:
+                BE.born_prologue(self.__fields[cur], self.__work, self.__source_normalized, self.__chi,
+                                 self.__chi_kind, self.__coef, s_box=self.__source_box)
+            out, k, ev = self.__launch_iteration(cur, (cur + 1) % 3, 0)
         self.__work_valid = True
-        d2, e2 = self.__norms.tolist()  # the one host sync of the iteration
+        # completes this iteration's successor on speculation: input `out`, output the third buffer
+        self.__pending = self.__launch_iteration(out, 3 - cur - out, 1 - k)
+        ev.synchronize()  # waits for this iteration only
+        d2, e2 = self.__norms_host[k].tolist()
         current = math.sqrt(d2)
         with np.errstate(divide='ignore', invalid='ignore'):
             rel = np.float64(current) / np.float64(self.__previous_update_norm)
         if rel < 1:
-            self.__field_array, self.__field_next = self.__field_next, self.__field_array
+            self.__cur = out
             self.__field_fp64 = None
             self.__previous_update_norm = current
             self.__field_norm = math.sqrt(e2)
@@ -492,7 +522,7 @@ class Solution:
         log.warning(f'The field update is scaled by {rel:0.3f} >= 1 in iteration {self.__iteration}.')
         alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)
         log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
-        self.__update_operators(alpha_new)  # also marks the speculative first pass in the work buffer stale
+        self.__update_operators(alpha_new)  # also voids the iteration launched ahead and the work buffer
         return False
 
     # ------------------------------------------------------------------ FP64 refinement
@@ -538,7 +568,7 @@ class Solution:
     def field_array_fp64(self) -> torch.Tensor:
         """The field in double precision: the refined field after ``refine()``, else the field converted."""
         f = self.__field_fp64
-        return f if f is not None else self.__field_array.to(torch.complex128)
+        return f if f is not None else self.__fields[self.__cur].to(torch.complex128)
 
     def refine(self, tol: float = 3e-5, max_iter: int = 10000, steps_per_sync: int = 16) -> dict:
         """One mixed-precision refinement sweep (no reference counterpart; the reference iterates in one precision).
@@ -559,8 +589,8 @@ class Solution:
                                                                       steps_per_sync=steps_per_sync, graph=False)
         E = E + bt.field_arrays()[0].to(torch.complex128)
         self.__field_fp64 = E
-        self.__field_array.copy_(E.to(self.__field_array.dtype))
-        self.__work_valid = False
+        self.__fields[self.__cur].copy_(E.to(self.__fields[self.__cur].dtype))
+        self.__invalidate_work()
         self.__field_norm = None
         self.__relative_residue = None
         s_c, chi = self.__fp64_residual_source(E)
@@ -574,7 +604,7 @@ class Solution:
         current field and the iteration bookkeeping."""
         return {'chi': self.__chi, 'chi_kind': self.__chi_kind, 's': self.__source_normalized,
                 's_box': self.__source_box,
-                'coef': self.__coef, 'field': self.__field_array, 'alpha': self.__alpha, 'alpha_base': self.__alpha_base,
+                'coef': self.__coef, 'field': self.__fields[self.__cur], 'alpha': self.__alpha, 'alpha_base': self.__alpha_base,
                 'beta': self.__beta, 'coef_a': self.__coef_a, 'coef_c': self.__coef_c,
                 'prev': self.__previous_update_norm, 'iteration': self.__iteration,
                 'ncomp': 1 + 2 * self.vectorial}
@@ -597,7 +627,7 @@ class Solution:
     def _batch_result(self, field: torch.Tensor, iteration: int, prev: float, field_norm: float, alpha: complex,
                       coef_a: float, coef_c: complex):
         """Take over the state a batched solve reached for this problem."""
-        self.__field_array.copy_(field)
+        self.__fields[self.__cur].copy_(field)
         self.__field_fp64 = None
         self.__iteration = int(iteration)
         self.__previous_update_norm = float(prev)
