@@ -177,6 +177,18 @@ int mmx_batch_step(mmx_plan_t plan, void* EA, void* EB, void* W, const void* s, 
 int mmx_wall_clock(double* out, void* stream);
 int mmx_wall_clock_rate_khz(int64_t* khz);
 
+/*
+ * Preconditioner set-up reductions over a complex128 permittivity eps ([1,1,grid] for eps_kind 1, [3,3,grid]
+ * row-major for eps_kind 2), whatever the plan's dtype; `out` is 2 device doubles.
+ *   mmx_setup_sigma: out[1] = max over voxels of the largest eigenvalue of (eps - shift I)^H (eps - shift I)
+ *                    (calc_DeltaEE_beta2 / calc_sigmaEE, solver.py:392-401, shift = alpha_r beta real);
+ *   mmx_setup_gain:  out[0], out[1] = min and max over voxels of the eigenvalues of -i/2 (eps - eps^H)
+ *                    (has_gain / has_loss, solver.py:322-333).
+ * Closed-form 3x3 Hermitian eigenvalues in FP64 (the reference's mat3_eigh, backend/__init__.py:692-850).
+ */
+int mmx_setup_sigma(mmx_plan_t plan, const void* eps, int eps_kind, double shift, double* out, void* stream);
+int mmx_setup_gain(mmx_plan_t plan, const void* eps, int eps_kind, double* out, void* stream);
+
 /* out[0] = sum |x|^2 over n complex elements, FP64 accumulation (backend/__init__.py:872-874, torch.py:316-317). */
 int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stream);
 

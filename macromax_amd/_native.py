@@ -19,7 +19,8 @@ MMX_C128 = 1
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
             'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue',
             'mmx_born_iterate',
-            'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_norm2', 'mmx_chi_mul',
+            'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_setup_sigma', 'mmx_setup_gain',
+            'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
 
 # batched-solve state bits and per-problem track record (include/macromax_hip.h, MMX_STATE_* / MMX_TRACK_*)
@@ -71,6 +72,10 @@ def _declare(L):
     L.mmx_wall_clock.argtypes = [c_p, c_p]
     L.mmx_wall_clock_rate_khz.restype = c_int
     L.mmx_wall_clock_rate_khz.argtypes = [ctypes.POINTER(c_i64)]
+    L.mmx_setup_sigma.restype = c_int
+    L.mmx_setup_sigma.argtypes = [c_p, c_p, c_int, ctypes.c_double, c_p, c_p]
+    L.mmx_setup_gain.restype = c_int
+    L.mmx_setup_gain.argtypes = [c_p, c_p, c_int, c_p, c_p]
     L.mmx_norm2.restype = c_int
     L.mmx_norm2.argtypes = [c_p, c_p, c_i64, c_p, c_p]
     L.mmx_chi_mul.restype = c_int

@@ -217,6 +217,33 @@ class Plan:
                                                state.data_ptr(), track.data_ptr(), norms.data_ptr(), float(tol),
                                                int(max_iter), _stream_ptr(self.device)), 'mmx_batch_step')
 
+    def _setup_eps(self, eps):
+        if not isinstance(eps, torch.Tensor) or eps.dtype != torch.complex128 or eps.device != self.device:
+            raise TypeError('set-up reductions take a complex128 device tensor')
+        eps = eps.contiguous()
+        if eps.numel() == self.V:
+            return eps, 1
+        if eps.numel() == 9 * self.V:
+            return eps, 2
+        raise ValueError('eps must be [1, 1, *grid] or [3, 3, *grid]')
+
+    def setup_sigma(self, eps, shift: float) -> float:
+        """max over voxels of the largest eigenvalue of (eps - shift I)^H (eps - shift I) (``mmx_setup_sigma``)."""
+        eps, kind = self._setup_eps(eps)
+        out = torch.empty(2, dtype=torch.float64, device=self.device)
+        _native.check(self._lib.mmx_setup_sigma(self.handle, eps.data_ptr(), kind, float(shift), out.data_ptr(),
+                                                _stream_ptr(self.device)), 'mmx_setup_sigma')
+        return float(out[1])
+
+    def setup_gain(self, eps):
+        """(min, max) over voxels of the eigenvalues of -i/2 (eps - eps^H) (``mmx_setup_gain``)."""
+        eps, kind = self._setup_eps(eps)
+        out = torch.empty(2, dtype=torch.float64, device=self.device)
+        _native.check(self._lib.mmx_setup_gain(self.handle, eps.data_ptr(), kind, out.data_ptr(),
+                                               _stream_ptr(self.device)), 'mmx_setup_gain')
+        lo, hi = out.tolist()
+        return lo, hi
+
     def norm2(self, x, out):
         self._check(x, 'norm2 input')
         _native.check(self._lib.mmx_norm2(self.handle, x.data_ptr(), x.numel(), out.data_ptr(),

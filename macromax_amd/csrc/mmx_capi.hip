@@ -9,6 +9,7 @@
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
 #include "mmx_regkernels.hpp"
+#include "mmx_setup.hpp"
 
 using namespace mmx;
 
@@ -506,7 +507,7 @@ int last_pass_tiles(const mmx_plan_s* p, const AxisPlan& inner, int C, int64_t n
 
 // norms[2b + {0,1}] = sums of the last pass's per-tile partials of problem b, in tile order.
 int reduce_norms(const double* partials, int64_t tiles, int64_t nbatch, double* norms, hipStream_t stream) {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nbatch), dim3(512), 0, stream, partials, tiles, norms);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)nbatch), dim3(1024), 0, stream, partials, tiles, norms);
     MMX_HIP_CHECK(hipGetLastError());
     return MMX_OK;
 }
@@ -1014,6 +1015,40 @@ int mmx_wall_clock_rate_khz(int64_t* khz) {
     MMX_HIP_CHECK(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev));
     *khz = rate;
     return MMX_OK;
+}
+
+// Set-up extremes over a complex128 permittivity (mode 0: sigma^2 with a shift; 1: gain eigenvalue range).
+int setup_extreme(mmx_plan_s* p, const void* eps, int full3, double shift, int mode, double* out, hipStream_t st) {
+    const int64_t V = p->V;
+    int nblk = (int)((V + 255) / 256);
+    if (nblk > 2048) nblk = 2048;
+    if (nblk < 1) nblk = 1;
+    const int rc = ensure_scratch(p, (size_t)2 * nblk * sizeof(double));
+    if (rc) return rc;
+    double* partials = reinterpret_cast<double*>(p->scratch);
+    const cx<double>* e = reinterpret_cast<const cx<double>*>(eps);
+    if (mode == 0)
+        hipLaunchKernelGGL((setup_extreme_kernel<0>), dim3(nblk), dim3(256), 0, st, e, V, full3, shift, partials);
+    else
+        hipLaunchKernelGGL((setup_extreme_kernel<1>), dim3(nblk), dim3(256), 0, st, e, V, full3, shift, partials);
+    MMX_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(setup_extreme_final_kernel, dim3(1), dim3(256), 0, st, partials, nblk, out);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+int mmx_setup_sigma(mmx_plan_t p, const void* eps, int eps_kind, double shift, double* out, void* stream) {
+    g_last_error.clear();
+    if (!p || !eps || !out) return fail(MMX_EINVAL, "null argument");
+    if (eps_kind != 1 && eps_kind != 2) return fail(MMX_EINVAL, "eps_kind must be 1 (isotropic) or 2 (3x3)");
+    return setup_extreme(p, eps, eps_kind == 2, shift, 0, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+int mmx_setup_gain(mmx_plan_t p, const void* eps, int eps_kind, double* out, void* stream) {
+    g_last_error.clear();
+    if (!p || !eps || !out) return fail(MMX_EINVAL, "null argument");
+    if (eps_kind != 1 && eps_kind != 2) return fail(MMX_EINVAL, "eps_kind must be 1 (isotropic) or 2 (3x3)");
+    return setup_extreme(p, eps, eps_kind == 2, 0.0, 1, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int mmx_norm2(mmx_plan_t p, const void* x, int64_t n, double* out, void* stream) {

@@ -464,15 +464,32 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
 }
 
 // Sum of per-tile partials -> out[2*b + {0,1}], fixed order (deterministic). One block per problem.
-__global__ void __launch_bounds__(512) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
+__global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
     __shared__ double red[16];
     const int64_t b = blockIdx.x;
-    const double* pp = partials + 2 * b * tiles;
-    double s0 = 0.0, s1 = 0.0;
-    for (int64_t q = threadIdx.x; q < tiles; q += blockDim.x) {
-        s0 += pp[2 * q];
-        s1 += pp[2 * q + 1];
+    const double2* pp = reinterpret_cast<const double2*>(partials + 2 * b * tiles);
+    const int64_t nt = blockDim.x;
+    // eight independent partial sums per thread so that all its loads are in flight together; the order of the
+    // additions depends only on the tile count and the block size
+    double2 acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
+    int64_t q = threadIdx.x;
+    for (; q + 7 * nt < tiles; q += 8 * nt) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pp[q + u * nt];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) { acc[u].x += v[u].x; acc[u].y += v[u].y; }
     }
+    for (int u = 0; q < tiles; q += nt, ++u) {
+        const double2 v = pp[q];
+        acc[u & 7].x += v.x;
+        acc[u & 7].y += v.y;
+    }
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) { s0 += acc[u].x; s1 += acc[u].y; }
     s0 = block_sum(s0, red);
     s1 = block_sum(s1, red);
     if (threadIdx.x == 0) {

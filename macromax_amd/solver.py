@@ -158,30 +158,21 @@ class Solution:
             raise ValueError('an anisotropic permittivity needs a vectorial calculation')
         eps64 = eps.to(torch.complex128)
         max_allowed_gain = math.sqrt(BE.eps)
-
-        def herm_part_min(a):  # smallest eigenvalue of -0.5i (a - a^H)
-            if a.shape[0] == 1:
-                return float(torch.amin((-0.5j * (a - a.conj())).real))
-            h = -0.5j * (a - a.transpose(0, 1).conj())
-            return float(torch.amin(_eigvals(h)[0]))
-
-        gain = herm_part_min(eps64) < -max_allowed_gain
-        loss = herm_part_min(-eps64) < -max_allowed_gain
+        # the set-up reductions run as FP64 kernels over the full grid (solver.py:313-401)
+        full = tuple(int(_) for _ in self.grid.shape)
+        eps_full = eps64.expand(eps64.shape[0], eps64.shape[1], *full).contiguous()
+        plan = BE.plan
+        h_min, h_max = plan.setup_gain(eps_full)  # eigenvalues of -i/2 (eps - eps^H)
+        gain = h_min < -max_allowed_gain          # has_gain(eps), solver.py:322-327
+        loss = -h_max < -max_allowed_gain         # has_gain(-eps): the eigenvalues change sign
         transpose = gain and not loss
         if gain and not transpose:
             log.warning('Convergence not guaranteed: the permittivity has gain.')
 
         beta = float(np.real(1.0 / self.__mu))
 
-        def sigma_ee(alpha_r):
-            ab = float(alpha_r) * beta
-            if eps64.shape[0] == 1:
-                return float(torch.amax((eps64 - ab).abs())) / abs(beta)
-            d = eps64 - ab * torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(3, 3, *([1] * self.grid.ndim))
-            dh = d.transpose(0, 1).conj()
-            delta2 = torch.einsum('ij...,jk...->ik...', dh, d)
-            lam = _eigvals(delta2)
-            return float(torch.amax(lam.abs())) ** 0.5 / abs(beta)
+        def sigma_ee(alpha_r):  # calc_sigmaEE, solver.py:392-401
+            return plan.setup_sigma(eps_full, float(alpha_r) * beta) ** 0.5 / abs(beta)
 
         alpha_real, min_value = scipy.optimize.fmin(lambda a: sigma_ee(np.asarray(a).ravel()[0]), 0.0,
                                                     initial_simplex=[[0.0], [1.0]], disp=False, full_output=True,
@@ -189,6 +180,7 @@ class Solution:
         alpha = complex(alpha_real[0] + 1.0j * min_value)
         if transpose:
             alpha = alpha.conjugate()
+        del eps_full
         alpha = self.__increase_bias_to_limit_kernel_width(alpha)
         log.info(f'Preconditioner constants: alpha = {alpha.real:0.4g} + {alpha.imag:0.4g}i, beta = {beta:0.4g}')
         self.__alpha = alpha
