@@ -57,7 +57,7 @@ def _gpu_gradient(kind, dtype, tol):
                     info=info)
     loss = _loss(E, box)
     loss.backward()
-    return eps_t.grad.numpy(), float(loss), info
+    return eps_t.grad.detach().numpy(), float(loss.detach()), info
 
 
 @pytest.mark.parametrize('kind', ['scalar2d', 'iso3d', 'aniso3d'])
