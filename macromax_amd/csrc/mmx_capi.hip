@@ -835,12 +835,12 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
     for (int a = 0; a < 3; ++a) {
         const int64_t n = p->shape3[a];
         if (n == 1) continue;
-        if (n > kMaxN) { delete p; return fail(MMX_EUNSUP, "axis longer than 4096 samples"); }
+        if (n > kMaxN) { mmx_plan_destroy(p); return fail(MMX_EUNSUP, "axis longer than 4096 samples"); }
         AxisPlan& ax = p->ax[p->naxes++];
         ax.orig = a;
         ax.N = (int)n;
         if (!factorize(ax.N, ax.rad, ax.nrad)) {
-            delete p;
+            mmx_plan_destroy(p);  // frees the twiddle tables of the axes before this one
             return fail(MMX_EUNSUP, "FFT length " + std::to_string(n) + " has a prime factor above 7");
         }
         int64_t inner = 1, outer = 1;
