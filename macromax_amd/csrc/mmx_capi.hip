@@ -1,4 +1,5 @@
 // C ABI of libmacromax_hip.so: plans, pass scheduling and launches (declarations in include/macromax_hip.h).
+#include <atomic>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -272,16 +273,30 @@ bool reg_supported(const AxisPlan& ax) {
 
 int reg_threads_per_pencil(int N, int NC, int kind) { return N / reg_elems(N, NC, kind); }
 
+// Dynamic LDS beyond 64 KiB has to be enabled per kernel (up to the 160 KiB of a gfx950 CU).
+template <auto KERNEL>
+void allow_lds(size_t lds) {
+    if (lds <= 65536) return;
+    static std::atomic<size_t> raised{0};  // per kernel: the largest dynamic LDS size enabled so far
+    if (lds > raised.load()) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(KERNEL), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        raised.store(lds);
+    }
+}
+
 template <int N, int KIND, int NC, int CK>
 void launch_reg(const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
     constexpr int E = reg_elems(N, NC, KIND);
     if constexpr (KIND == kEpiPro || KIND == kGreenEpiPro) {
         if (a.s_restricted) {
+            allow_lds<&reg_pass_kernel<float, N, E, KIND, NC, CK, 1>>(t.lds);
             hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 1>), dim3((unsigned)t.nblocks),
                                dim3(t.nth), t.lds, stream, a);
             return;
         }
     }
+    allow_lds<&reg_pass_kernel<float, N, E, KIND, NC, CK, 0>>(t.lds);
     hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 0>), dim3((unsigned)t.nblocks), dim3(t.nth),
                        t.lds, stream, a);
 }
@@ -358,7 +373,8 @@ int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, i
         t.tiles_per_problem = t.nouter * t.ntile_inner;
     }
     t.nth = t.P * TP;
-    t.lds = (size_t)(ax.N + t.P * (ax.N + ax.N / 16)) * sizeof(cx<float>);
+    const int E = ax.N / TP;
+    t.lds = (size_t)(ax.N + (size_t)t.P * padded_len(ax.N, E)) * sizeof(cx<float>);
     t.nblocks = t.tiles_per_problem * mult;
     return MMX_OK;
 }

@@ -73,10 +73,41 @@ constexpr int num_stages(int N, int E) {
     while (p < N) { p *= stage_radix(N, E, s); ++s; }
     return s;
 }
+// Twiddle table in LDS, laid out per stage: stage s >= 1 (radix R, P = product of earlier radices) keeps
+// w^(j k N/(P R)) for j = 1..R-1, k = 0..P-1 at tw_stage_off(s) + (j-1) P + k, so that the lanes of a wave, which
+// differ in k, read consecutive entries. The stages together hold N - 1 entries.
+constexpr int tw_stage_off(int N, int E, int s) {
+    int off = 0;
+    for (int q = 1; q < s; ++q) off += (stage_radix(N, E, q) - 1) * stage_prod(N, E, q);
+    return off;
+}
 
-// LDS position with one pad slot every 16 entries (breaks the power-of-two strides of the exchange).
-__device__ __forceinline__ int padpos(int pos) { return pos + (pos >> 4); }
-constexpr int padded_len(int N) { return N + N / 16; }
+// Exponent (in units of 2 pi / N) of staged-table entry e, for the stages from S on.
+template <int N, int E, int S>
+__device__ __forceinline__ int tw_stage_exponent(int e) {
+    if constexpr (S >= num_stages(N, E)) {
+        return 0;
+    } else {
+        constexpr int P = stage_prod(N, E, S), R = stage_radix(N, E, S), off = tw_stage_off(N, E, S);
+        if (e < off + (R - 1) * P) return (1 + (e - off) / P) * ((e - off) & (P - 1)) * (N / (P * R));
+        return tw_stage_exponent<N, E, S + 1>(e);
+    }
+}
+
+// Fill the LDS twiddle table (N entries) from the natural-order table g[i] = exp(-2 pi i I / N) in global memory.
+template <typename T, int N, int E>
+__device__ __forceinline__ void load_twiddles(cx<T>* tw, const cx<T>* g, int tid, int nth) {
+    for (int e = tid; e < N - 1; e += nth) tw[e] = g[tw_stage_exponent<N, E, 1>(e)];
+}
+
+// LDS position of exchange entry `pos`: one pad slot every 2^pad_shift(E) entries, which breaks the power-of-two
+// strides of the exchange. LDS has 64 four-byte banks; an 8-byte read serves 32 lanes per cycle and an 8-byte write
+// 16. With E = 8 (three radix-8/4 stages) a slot every 8 entries makes the strided-tile exchange and the rows-tile
+// writes conflict-free (rows-tile reads then pair up 3 of 32 lanes); with E = 16 a slot every 16 is conflict-free.
+constexpr int pad_shift(int E) { return E == 8 ? 3 : 4; }
+template <int E> __device__ __forceinline__ int padpos(int pos) { return pos + (pos >> pad_shift(E)); }
+constexpr int padded_len(int N, int E) { return N + (N >> pad_shift(E)); }
+
 
 // One stage on the registers v[NC][E] of all NC components.
 template <typename T, int N, int E, int S, int DIR, int NC>
@@ -85,7 +116,6 @@ __device__ __forceinline__ void reg_stage(cx<T> (&v)[NC][E], int t, const cx<T>*
     constexpr int P = stage_prod(N, E, S);
     constexpr int TP = N / E;
     constexpr int BQ = E / R;
-    constexpr int TWS = N / (P * R);
 #pragma unroll
     for (int q = 0; q < BQ; ++q) {
         const int i = t + q * TP;
@@ -94,7 +124,7 @@ __device__ __forceinline__ void reg_stage(cx<T> (&v)[NC][E], int t, const cx<T>*
         if constexpr (P > 1) {
 #pragma unroll
             for (int j = 1; j < R; ++j) {
-                w[j] = tw[j * k * TWS];
+                w[j] = tw[tw_stage_off(N, E, S) + (j - 1) * P + k];
                 if (DIR > 0) w[j].i = -w[j].i;
             }
         }
@@ -115,9 +145,10 @@ __device__ __forceinline__ void reg_stage(cx<T> (&v)[NC][E], int t, const cx<T>*
 }
 
 // Exchange after stage S: outputs to their Stockham positions in LDS, then read back positions t + m*TP.
-// `xb` is this pencil's exchange buffer (padded_len(N) entries, with stride `ps` between entries: 1 for
+// `xb` is this pencil's exchange buffer (padded_len(N, E) entries, with stride `ps` between entries: 1 for
 // pencil-major buffers, the pencil count for interleaved ones). All NC components go through the buffer one
-// after the other.
+// after the other (one buffer per pencil keeps the LDS per block small enough for three blocks per CU; an NC-buffer
+// exchange with one barrier pair per stage measured 3 % slower at 256^3).
 template <typename T, int N, int E, int S, int NC>
 __device__ __forceinline__ void reg_exchange(cx<T> (&v)[NC][E], int t, cx<T>* xb, int ps) {
     constexpr int R = stage_radix(N, E, S);
@@ -132,11 +163,11 @@ __device__ __forceinline__ void reg_exchange(cx<T> (&v)[NC][E], int t, cx<T>* xb
             const int k = i & (P - 1);
             const int base = (i - k) * R + k;
 #pragma unroll
-            for (int j = 0; j < R; ++j) xb[padpos(base + j * P) * ps] = v[c][q + j * BQ];
+            for (int j = 0; j < R; ++j) xb[padpos<E>(base + j * P) * ps] = v[c][q + j * BQ];
         }
         __syncthreads();
 #pragma unroll
-        for (int m = 0; m < E; ++m) v[c][m] = xb[padpos(t + m * TP) * ps];
+        for (int m = 0; m < E; ++m) v[c][m] = xb[padpos<E>(t + m * TP) * ps];
         __syncthreads();
     }
 }

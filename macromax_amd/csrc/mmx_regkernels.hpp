@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     int p, t;
     cx<T>* xb;
     int ps;
-    if (rows) { t = tid & (TP - 1); p = tid >> LTP; xb = xbuf + p * padded_len(N); ps = 1; }
+    if (rows) { t = tid & (TP - 1); p = tid >> LTP; xb = xbuf + p * padded_len(N, E); ps = 1; }
     else { p = tid & (P - 1); t = tid >> LP; xb = xbuf + p; ps = P; }
 
     int64_t r0 = 0, o = 0, ib = 0;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         }
     }
 
-    for (int q = tid; q < N; q += blockDim.x) tw[q] = a.tw[q];
+    load_twiddles<T, N, E>(tw, a.tw, tid, blockDim.x);
     __syncthreads();
 
     cx<T> v[NC][E];
