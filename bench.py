@@ -227,7 +227,75 @@ def batched_benchmark(tol=1e-6, max_iter=10000, n_problems=64, nrhs=256, seq_rhs
         'iterations_median': int(np.median(its_b)), 'iterations_max': int(its_b.max()), 'autotune_s': round(tune_b, 2),
         'iterations_equal_to_sequential': bool(all(its_eq)), 'max_rel_l2_vs_sequential': float(max(errs)),
         'timed': 'BatchSolution.from_sources(...).solve end to end; sequential: Solution.solve per right-hand side'}
+    del bt, Eb
+    torch.cuda.empty_cache()
+    out['part_c'] = smatrix_benchmark(log=log)
     return out
+
+
+def smatrix_benchmark(n0=512, nt=128, tol=1e-6, log=lambda m: None):
+    """Part C: the scattering matrix of a 2D scalar multilayer thin film (seven one-pixel layers of index 1.5, the
+    reference's unitarity case, python/tests/test_matrix.py:249-262) on an n0 x nt grid at 125 nm, 500 nm
+    wavelength, 20-wavelength LinearBound along the propagation axis, periodic across it; complex128 and residue 1e-6
+    for every column (the reference's ScatteringMatrix defaults). A layer with structure across the propagation axis
+    would trap light travelling along it (nothing absorbs across), so that its columns do not reach 1e-6 within 10^4
+    iterations. ScatteringMatrix.__array__ with one batched solve of all columns against the reference's
+    column-by-column Solution loop (the same class with a callback), and the complex128 NumPy oracle iteration on the
+    host, extrapolated to the columns' total iteration count."""
+    import macromax_amd as mm
+    from macromax_amd import matrix
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    wavelength = 500e-9
+    grid = mm.Grid([n0, nt], wavelength / 4)
+    thickness = np.zeros([2, 1])
+    thickness[0, 0] = 20 * wavelength
+    bound = LinearBound(grid, thickness=thickness, max_extinction_coefficient=0.25)
+    n = np.ones(grid.shape)
+    for off in (-11, -10, -9, -8, -1, 0, 2):
+        n[n0 // 2 + off] = 1.5
+    kw = dict(vectorial=False, vacuum_wavelength=wavelength, refractive_index=n, bound=bound, dtype=np.complex128)
+    m_b = matrix.ScatteringMatrix(grid, tol=tol, **kw)
+    arr_b, t_b = _timed(lambda: np.asarray(m_b))
+    log(f'part C batched scattering matrix {arr_b.shape}: {t_b:.2f} s')
+    seq_its = [0]
+
+    def seq_callback(s_):
+        seq_its[0] += 1
+        return s_.iteration < 1e4 and s_.residue > tol
+    m_s = matrix.ScatteringMatrix(grid, callback=seq_callback, **kw)
+    arr_s, t_s = _timed(lambda: np.asarray(m_s))
+    log(f'part C column-by-column: {t_s:.2f} s')
+    sv = np.linalg.svd(arr_b, compute_uv=False)
+    # iterations of every column (one more batched solve of the basis, for the count only; not timed)
+    from macromax_amd.batch import BatchSolution
+    eye = np.eye(arr_b.shape[1], dtype=np.complex128)
+    srcs = m_b._sources(m_b._vectors(eye))
+    bt = BatchSolution.from_sources(m_b.solution, sources=srcs)
+    bt.solve(tol=tol, max_iter=10000)
+    total_its = int(np.sum(bt.iteration))
+    sol = m_b.solution
+    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    s0 = srcs[0].cpu().numpy().astype(np.complex128)[:, None] * (1.0j / sol.alpha.imag / sol.beta)
+    k = ob.k_vectors((n0, nt), np.asarray(grid.step) * sol.wavenumber)
+    o = ob.BornOracle(chi, s0, sol.alpha, sol.beta, k, False)
+    o.step()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        o.step()
+    t_cpu_it = (time.perf_counter() - t0) / 5
+    return {'grid': [n0, nt], 'columns': int(arr_b.shape[1]), 'dtype': 'c128', 'residue': tol,
+            'medium': 'seven one-pixel layers of index 1.5 around the centre',
+            'seconds': round(t_b, 3), 'sequential_seconds': round(t_s, 3), 'speedup_vs_sequential': round(t_s / t_b, 2),
+            'max_rel_l2_vs_sequential': _rel_l2(arr_b, arr_s),
+            'singular_values_min_max': [round(float(sv.min()), 4), round(float(sv.max()), 4)],
+            'column_iterations_total': total_its, 'column_iterations_max': int(np.max(bt.iteration)),
+            'sequential_iterations_total': int(seq_its[0]),
+            'cpu_oracle_seconds_per_iteration': round(t_cpu_it, 5),
+            'cpu_oracle_extrapolated_seconds': round(t_cpu_it * total_its, 1),
+            'timed': 'ScatteringMatrix.__array__ end to end (sources, batched solve, detection); sequential: the '
+                     'same class with a callback, one Solution.solve per column; medium set-up excluded; CPU: '
+                     'one complex128 oracle iteration (numpy.fft, 1 core) x the columns\' total iterations'}
 
 
 # ---------------------------------------------------------------------------------------------------------------

@@ -549,6 +549,22 @@ class BackEndHip(BackEnd):
     def curl(self, field_array) -> torch.Tensor:
         return self.ift(self.curl_ft(self.ft(field_array)))
 
+    def div_ft(self, field_array_ft) -> torch.Tensor:
+        """sum_d i k_d F[d] over the spatial axes, [m, n, *grid] -> [n, 1, *grid] (``backend/__init__.py:555-569``)."""
+        F = self.astype(field_array_ft)
+        out = torch.zeros(F.shape[1:], dtype=self.hardware_dtype, device=self.device)
+        for d in range(min(self.grid.ndim, F.shape[0])):
+            out = out + 1j * self.k[d].to(self.hardware_dtype) * F[d]
+        return out.unsqueeze(1)
+
+    def div(self, field_array) -> torch.Tensor:
+        """Divergence of a vector or tensor field [m, n, *grid] -> [n, 1, *grid] (HIP FFTs)."""
+        return self.ift(self.div_ft(self.ft(field_array)))
+
+    def outer(self, A, B) -> torch.Tensor:
+        """Dyadic product A B^H of [3, 1, *grid] fields, [3, 3, *grid] (``backend/__init__.py:531-542``)."""
+        return self.astype(A) * self.conj(self.astype(B))[None, :, 0]
+
     def cross(self, A, B):
         A = self.astype(A)
         B = self.astype(B)

@@ -406,33 +406,79 @@ class Solution:
         """The k0^2-scaled field on the device, [C, 1, *grid] (read-only view of the current buffer)."""
         return self.__fields[self.__cur]
 
+    # ------------------------------------------------------------------ derived fields (solver.py:834-995)
+    # The backend's derivatives are with respect to k0 x (its grid is in units of 1/k0), as in the reference.
     @property
     def B(self) -> np.ndarray:
+        """Magnetic field B = curl E / (i c) in SI units, [3, *grid] (``solver.py:834-845``)."""
         BE = self.__BE
         B = BE.curl(BE.astype(self.E[:, np.newaxis])) / (1.0j * const.c)
         return BE.asnumpy(B)[:, 0]
 
     @property
     def H(self) -> np.ndarray:
+        """Magnetising field H = mu^-1 B / mu_0 (non-magnetic media), [3, *grid] (``solver.py:862-889``)."""
         BE = self.__BE
         mu_H = (-1.0j / (const.mu_0 * const.c)) * BE.curl(BE.astype(self.E[:, np.newaxis]))
-        return BE.asnumpy(mu_H * self.__beta)[:, 0]
+        return BE.asnumpy(mu_H * self.__beta)[:, 0]  # beta = 1 / mu for non-magnetic media
 
     @property
     def D(self) -> np.ndarray:
+        """Displacement field D = i (k0 curl H - J) / omega, J = S / (-i omega mu_0), [3, *grid]
+        (``solver.py:847-860``)."""
         BE = self.__BE
         D = BE.curl(BE.astype(self.H[:, np.newaxis])) * self.wavenumber
         S = BE.astype(self.source_distribution[:, np.newaxis])
-        D = D - (1.0 / (1.0j * self.angular_frequency * const.mu_0)) * S * self.__beta
+        D = D - (1.0 / (1.0j * self.angular_frequency * const.mu_0)) * S
         D = D * (1j / self.angular_frequency)
         return BE.asnumpy(D)[:, 0]
 
     @property
     def S(self) -> np.ndarray:
+        """Time-averaged Poynting vector Re(E x H*) / 2, [3, *grid] (``solver.py:891-903``)."""
         BE = self.__BE
         E = BE.astype(self.E[:, np.newaxis])
         H = BE.astype(self.H[:, np.newaxis])
         return (0.5 * BE.asnumpy(BE.cross(E, H.conj())).real)[:, 0]
+
+    @property
+    def divS(self) -> np.ndarray:
+        """Divergence of S with respect to k0 x (the reference's backend units), [*grid] (``solver.py:905-912``)."""
+        BE = self.__BE
+        return BE.asnumpy(BE.div(BE.astype(self.S[:, np.newaxis]))[0, 0])
+
+    @property
+    def energy_density(self) -> np.ndarray:
+        """Time-averaged energy density (E.D* + B.H*) / 4, [*grid] (``solver.py:914-930``)."""
+        E, B, H, D = self.E, self.B, self.H, self.D
+        u = np.sum((E * D.conj()).real, axis=0) + np.sum((B * H.conj()).real, axis=0)
+        return u * 0.25
+
+    @property
+    def stress_tensor(self) -> np.ndarray:
+        """Maxwell stress tensor (eps_0 E E* + mu_0 H H* - I (eps_0 |E|^2 + mu_0 |H|^2) / 2) / 2, real part,
+        [3, 3, *grid] (``solver.py:932-953``)."""
+        BE = self.__BE
+        E = BE.astype(self.E[:, np.newaxis])
+        H = BE.astype(self.H[:, np.newaxis])
+        E2 = torch.sum(E.abs() ** 2, dim=0)
+        H2 = torch.sum(H.abs() ** 2, dim=0)
+        sigma = const.epsilon_0 * BE.outer(E, E) + const.mu_0 * BE.outer(H, H)
+        sigma = sigma - (0.5 * BE.eye) * (const.epsilon_0 * E2 + const.mu_0 * H2)
+        return BE.asnumpy(0.5 * sigma).real
+
+    @property
+    def f(self) -> np.ndarray:
+        """Electromagnetic force density k0 Re(div sigma) in N/m^3, [3, *grid] (``solver.py:955-970``)."""
+        BE = self.__BE
+        force = BE.real(BE.div(BE.astype(self.stress_tensor))) * self.wavenumber
+        return BE.asnumpy(force)[:, 0]
+
+    @property
+    def torque(self) -> np.ndarray:
+        """Off-diagonal stress components (sigma_yz, sigma_xz, sigma_xy), [3, *grid] (``solver.py:972-991``)."""
+        sigma = self.stress_tensor
+        return np.concatenate([sigma[1, 2, np.newaxis], sigma[0, 2, np.newaxis], sigma[0, 1, np.newaxis]])
 
     @property
     def iteration(self) -> int:

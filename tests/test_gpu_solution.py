@@ -220,3 +220,56 @@ def test_divergence_guard_matches_oracle():
         Eg = sol.field_array.cpu().numpy()
         assert np.linalg.norm((Eg - o.E).ravel()) <= 1e-9 * max(np.linalg.norm(o.E.ravel()), 1e-300)
     assert rejected >= 1
+
+
+def test_derived_fields_of_plane_wave():
+    """Derived fields (solver.py:834-991) of the solved vectorial plane wave, checked against what the physics fixes
+    for a y-polarised wave travelling along +x: in vacuum S_x = |E|^2 / (2 mu_0 c), D = eps_0 E,
+    u = eps_0 |E|^2 / 2, stress diag(-u, 0, 0) with zero off-diagonal (torque) entries; inside the absorbing layer
+    the force density equals d sigma_xx / dx and div S equals (d S_x / dx) / k0 (the reference's divS is taken
+    with respect to k0 x), both against central differences of the same arrays. Tolerance 2 % of the scale of each
+    quantity (complex64 solve to residue 1e-6); 5 % for the two derivatives (spectral against finite differences)."""
+    print('derived fields of the vectorial plane wave', flush=True)
+    sol, ref, sel = _plane_wave('vectorial')
+    E = sol.E
+    eps0, mu0, c = const.epsilon_0, const.mu_0, const.c
+    k0 = sol.wavenumber
+    x = np.asarray(sol.grid[0]).ravel()
+    Ey2 = np.abs(E[1, sel]) ** 2
+    S = sol.S
+    assert S.shape == E.shape
+    npt.assert_allclose(S[0, sel], Ey2 / (2 * mu0 * c), rtol=0, atol=2e-2 * np.max(Ey2) / (2 * mu0 * c))
+    assert np.max(np.abs(S[1:, sel])) <= 2e-2 * np.max(np.abs(S[0, sel]))
+    D = sol.D
+    npt.assert_allclose(D[1, sel], eps0 * E[1, sel], rtol=0, atol=2e-2 * eps0 * np.max(np.abs(E[1, sel])))
+    u = sol.energy_density
+    assert u.shape == E.shape[1:]
+    npt.assert_allclose(u[sel], 0.5 * eps0 * Ey2, rtol=0, atol=2e-2 * 0.5 * eps0 * np.max(Ey2))
+    sigma = sol.stress_tensor
+    assert sigma.shape == (3, 3, *E.shape[1:])
+    npt.assert_array_equal(sigma, sigma.transpose(1, 0, *range(2, sigma.ndim)))
+    scale = np.max(u[sel])
+    npt.assert_allclose(sigma[0, 0, sel], -u[sel], rtol=0, atol=2e-2 * scale)
+    assert np.max(np.abs(sigma[1, 1, sel])) <= 2e-2 * scale and np.max(np.abs(sigma[2, 2, sel])) <= 2e-2 * scale
+    torque = sol.torque
+    assert torque.shape == E.shape and np.max(np.abs(torque[:, sel])) <= 2e-2 * scale
+    # inside the absorbing layer (away from its start and from the periodic wrap); the spectral discretisation
+    # leaves a small alternating (Nyquist-frequency) tail on the field, which the derivatives amplify, so both
+    # sides are compared after averaging neighbouring samples
+    layer = (x > x[-1] - 9e-6) & (x < x[-1] - 2e-6)
+
+    def pairs(a):
+        a = a[..., layer]
+        return 0.5 * (a[..., 1:] + a[..., :-1])
+    force = sol.f
+    assert force.shape == E.shape
+    dsig = pairs(np.gradient(sigma[0, 0], x))
+    npt.assert_allclose(pairs(force[0]), dsig, rtol=0, atol=5e-2 * np.max(np.abs(dsig)))
+    assert np.max(np.abs(pairs(force[1:]))) <= 5e-2 * np.max(np.abs(dsig))
+    assert np.min(pairs(force[0])) > 0  # the absorbed wave pushes along +x
+    divS = sol.divS
+    assert divS.shape == E.shape[1:]
+    dS = pairs(np.gradient(S[0], x) / k0)
+    npt.assert_allclose(pairs(divS.real), dS, rtol=0, atol=5e-2 * np.max(np.abs(dS)))
+    assert np.max(pairs(divS.real)) < 0  # absorption
+    assert np.max(np.abs(divS[sel])) <= 5e-2 * np.max(np.abs(S[0, sel]))  # no sources or losses in vacuum
