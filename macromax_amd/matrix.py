@@ -221,7 +221,7 @@ class _Modes:
     def __init__(self, grid: Grid, k0: float, dtype):
         nd = grid.ndim
         tshape = tuple(int(_) for _ in grid.shape[1:])
-        k_t = [np.fft.fftshift(np.asarray(grid.k[a]).ravel()) for a in range(1, nd)]
+        k_t = [_centred(np.asarray(grid.k[a]).ravel()) for a in range(1, nd)]
         tol = np.sqrt(np.finfo(np.float64 if dtype is None else dtype).eps)
         propagating = np.ones(tshape, dtype=bool)
         k_rem = np.full(tshape, k0)
@@ -252,6 +252,11 @@ class _Modes:
     @property
     def count(self) -> int:
         return int(self.index.size)
+
+
+def _centred(v: np.ndarray) -> np.ndarray:
+    """A 1D array in fftfreq order reordered from the most negative frequency up (numpy's fftshift)."""
+    return np.roll(v, v.shape[0] // 2)
 
 
 def _unit(v: np.ndarray) -> np.ndarray:
@@ -547,7 +552,10 @@ class ScatteringMatrix(LiteralScatteringMatrix):
 
     def detvec2srcvec(self, vec: array_like) -> np.ndarray:
         """Time-reversed source vector of a detection vector: sides swapped, values conjugated (``:755-765``)."""
-        return np.fft.fftshift(np.asarray(vec).conj())
+        vec = np.asarray(vec).conj()
+        for axis in range(vec.ndim):
+            vec = np.roll(vec, vec.shape[axis] // 2, axis=axis)
+        return vec
 
     def _matvec(self, right: array_like) -> np.ndarray:
         right, shape = _as_columns(right)
