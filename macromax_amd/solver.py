@@ -118,34 +118,55 @@ class Solution:
         self.__BE64 = None
         self.E = initial_field
 
-        mu = func2arr(mu)
-        xi = func2arr(xi)
-        zeta = func2arr(zeta)
-        if np.any(np.asarray(xi) != 0) or np.any(np.asarray(zeta) != 0) or np.asarray(mu).size != 1:
-            raise NotImplementedError('Magnetic and bi-(an)isotropic media are not supported by this package.')
-        self.__mu = complex(np.asarray(mu).ravel()[0])
-        if isinstance(bound, Magnetic) and np.any(np.asarray(bound.magnetic_susceptibility) != 0):
-            raise NotImplementedError('Magnetic boundaries are not supported by this package.')
+        def media_field(f):  # mu, xi, zeta as complex128 [1|3, 1|3, *grid-broadcastable] device tensors
+            f = func2arr(f)
+            t = f if isinstance(f, torch.Tensor) else torch.as_tensor(np.asarray(f, dtype=np.complex128))
+            t = t.to(device=BE.device, dtype=torch.complex128)
+            while t.ndim < self.grid.ndim + 2:
+                t = t.unsqueeze(0)
+            return t.transpose(0, 1) if (t.shape[0] == 1 and t.shape[1] > 1) else t
+        mu = media_field(mu)
+        xi = media_field(xi)
+        zeta = media_field(zeta)
 
         if epsilon is None:
             n = func2arr(refractive_index) if refractive_index is not None else 1.0
             n = BE.to_matrix_field(np.asarray(n).astype(dtype))
             epsilon = BE.mul(n, n)
             if BE.is_scalar(n) and bool(torch.any(n.real < 0)):
-                raise NotImplementedError('Negative refractive indices imply a magnetic medium (unsupported).')
+                # a negative refractive index inverts both the permittivity and the permeability (solver.py:230-234)
+                sign = (1 - 2 * (n.real < 0).to(torch.float64)).to(torch.complex128)
+                mu = mu * sign
+                epsilon = epsilon * sign.to(epsilon.dtype)
         else:
             epsilon = func2arr(epsilon)
             epsilon = BE.astype(epsilon if isinstance(epsilon, torch.Tensor) else np.asarray(epsilon).astype(dtype))
 
-        # boundary (solver.py:242-247)
+        # boundary (solver.py:242-253)
         if isinstance(self.__bound, Electric):
             chi_b = BE.astype(self.__bound.electric_susceptibility)
             if epsilon.ndim >= 2 + self.grid.ndim and (epsilon.shape[0] > 1 or epsilon.shape[1] > 1):
                 chi_b = BE.eye * chi_b
             epsilon = BE.astype(BE.to_matrix_field(epsilon) + chi_b)
+        if isinstance(self.__bound, Magnetic):
+            chi_m = torch.as_tensor(np.asarray(self.__bound.magnetic_susceptibility, dtype=np.complex128),
+                                    device=BE.device)
+            mu = mu + torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(
+                3, 3, *([1] * self.grid.ndim)) * chi_m
 
+        # magnetic: mu not constant, or any bi-(an)isotropy (solver.py:304-307)
+        self.__magnetic = not (BE.allclose(xi) and BE.allclose(zeta) and
+                               bool(torch.allclose(mu, mu.reshape(-1)[0].expand_as(mu))))
+        self.__mag = None
         self.__source_normalized = None
-        self.__prepare_preconditioner(source_distribution, epsilon)
+        if self.__magnetic:
+            if not self.vectorial:
+                raise ValueError('magnetic and bi-(an)isotropic media need a vectorial calculation')
+            self.__mu = complex(1.0)
+            self.__prepare_magnetic(source_distribution, epsilon, xi, zeta, mu)
+        else:
+            self.__mu = complex(mu.reshape(-1)[0])
+            self.__prepare_preconditioner(source_distribution, epsilon)
         self.__relative_residue = None
 
     # ------------------------------------------------------------------ preconditioner (solver.py:270-493)
@@ -213,6 +234,35 @@ class Solution:
         self.__upload_coef()
         self.source_distribution = source_distribution
 
+    def __prepare_magnetic(self, source_distribution, epsilon, xi, zeta, mu):
+        """Preconditioner and operators of a magnetic medium (``solver.py:270-493``), ``magnetic.py``."""
+        from .magnetic import MagneticOperator
+        BE = self.__BE
+        eps = BE.to_matrix_field(epsilon).to(torch.complex128)
+        full = tuple(int(_) for _ in self.grid.shape)
+        max_allowed_gain = math.sqrt(BE.eps)
+        for name, f in (('permittivity', eps), ('xi', xi), ('zeta', zeta), ('permeability', mu)):
+            lo, _ = BE.plan.setup_gain(f.expand(f.shape[0], f.shape[1], *full).contiguous())
+            if lo < -max_allowed_gain:
+                log.warning(f'Convergence not guaranteed: the {name} has gain.')
+        self.__mag = MagneticOperator(BE, BE.plan, full, self.grid.step, self.wavenumber, eps, xi, zeta, mu,
+                                      self.__increase_bias_to_limit_kernel_width)
+        self.__alpha = self.__mag.alpha
+        self.__alpha_base = self.__alpha
+        self.__beta = self.__mag.beta
+        self.__eps = eps
+        self.__coef_a = 1.0
+        self.__coef_c = 0j
+        self.__chi_kind = 0
+        self.__chi = None
+        self.__coef = None
+        self.source_distribution = source_distribution
+
+    def __require_non_magnetic(self, what: str):
+        if self.__magnetic:
+            raise NotImplementedError(f'{what} is available for non-magnetic media only (the magnetic susceptibility '
+                                      f'includes curl operators).')
+
     def __upload_coef(self):
         self.__invalidate_work()
         a = self.__alpha
@@ -270,7 +320,8 @@ class Solution:
 
     @property
     def magnetic(self) -> bool:
-        return False
+        """True for media with a non-constant permeability or bi-(an)isotropy (``solver.py:304-307``)."""
+        return self.__magnetic
 
     @property
     def bound(self) -> Bound:
@@ -297,6 +348,7 @@ class Solution:
     @property
     def chi(self) -> torch.Tensor:
         """The susceptibility chi for the alpha chosen at set-up, as a [1,1,*grid] or [3,3,*grid] tensor."""
+        self.__require_non_magnetic('chi')
         if self.__chi_kind != 3:
             return self.__chi
         c = self.__chi
@@ -306,6 +358,7 @@ class Solution:
     @property
     def chi_effective(self) -> torch.Tensor:
         """chi for the current alpha: a chi + c I (the operator the kernels apply)."""
+        self.__require_non_magnetic('chi_effective')
         c = self.chi * self.__coef_a
         if self.__coef_c != 0:
             if c.shape[0] == 1:
@@ -317,6 +370,8 @@ class Solution:
     @property
     def source_normalized(self) -> torch.Tensor:
         """s = (i/(alpha_i beta)) S for the current alpha, [C, 1, *grid]."""
+        if self.__magnetic:
+            return self.__mag.s
         return self.__source_normalized * self.__coef_a
 
     @property
@@ -337,6 +392,11 @@ class Solution:
         S = BE.to_matrix_field(new_source)
         full = (1 + 2 * self.vectorial, 1, *[int(_) for _ in self.grid.shape])
         S = S.to(torch.complex128).expand(*full)
+        if self.__magnetic:
+            self.__mag.set_source(S)
+            self.__previous_update_norm = math.inf
+            self.__invalidate_work()
+            return
         self.__source_normalized = (S * (1.0j / self.__alpha_base.imag / self.__beta)).to(
             BE.hardware_dtype).contiguous()
         self.__source_box = source_support(self.__source_normalized, self.grid.ndim)
@@ -417,8 +477,10 @@ class Solution:
 
     @property
     def H(self) -> np.ndarray:
-        """Magnetising field H = mu^-1 B / mu_0 (non-magnetic media), [3, *grid] (``solver.py:862-889``)."""
+        """Magnetising field H = i (-mu^-1 curl E + i mu^-1 zeta E) / (mu_0 c), [3, *grid] (``solver.py:862-889``)."""
         BE = self.__BE
+        if self.__magnetic:
+            return BE.asnumpy(self.__mag.H(BE.astype(self.E[:, np.newaxis]), const.mu_0 * const.c))[:, 0]
         mu_H = (-1.0j / (const.mu_0 * const.c)) * BE.curl(BE.astype(self.E[:, np.newaxis]))
         return BE.asnumpy(mu_H * self.__beta)[:, 0]  # beta = 1 / mu for non-magnetic media
 
@@ -532,6 +594,8 @@ class Solution:
         The next iteration is enqueued before this one's norms are read (assuming acceptance, which holds for
         nearly every iteration), so the device never waits for the host's decision; it is discarded when the update
         is rejected."""
+        if self.__magnetic:
+            return self.__magnetic_step()
         self.__iteration += 1
         self.__relative_residue = None
         BE = self.__BE
@@ -561,6 +625,31 @@ class Solution:
         alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)
         log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
         self.__update_operators(alpha_new)  # also voids the iteration launched ahead and the work buffer
+        return False
+
+    def __magnetic_step(self) -> bool:
+        """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``)."""
+        self.__iteration += 1
+        self.__relative_residue = None
+        BE = self.__BE
+        E = self.__fields[self.__cur]
+        d = self.__mag.update(E)
+        current = BE.norm(d)
+        with np.errstate(divide='ignore', invalid='ignore'):
+            rel = np.float64(current) / np.float64(self.__previous_update_norm)
+        if rel < 1:
+            nxt = (self.__cur + 1) % 3
+            torch.add(E, d, out=self.__fields[nxt])
+            self.__cur = nxt
+            self.__field_fp64 = None
+            self.__previous_update_norm = current
+            self.__field_norm = BE.norm(self.__fields[nxt])
+            return True
+        log.warning(f'The field update is scaled by {rel:0.3f} >= 1 in iteration {self.__iteration}.')
+        alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)
+        log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
+        self.__mag.set_alpha(alpha_new)
+        self.__alpha = alpha_new
         return False
 
     # ------------------------------------------------------------------ FP64 refinement
@@ -617,6 +706,7 @@ class Solution:
         relative accuracy of the correction solve, so a complex64 solution is brought below the single-precision
         floor of the iteration itself. Returns the FP64 relative residues before and after and the iterations used.
         """
+        self.__require_non_magnetic('refine')
         from .batch import BatchSolution
         E = self.field_array_fp64.clone()
         s_c, chi = self.__fp64_residual_source(E)
@@ -640,6 +730,7 @@ class Solution:
     def _batch_operands(self) -> dict:
         """Device operands for a batched solve: chi and s for the set-up alpha, the affine coefficients, the
         current field and the iteration bookkeeping."""
+        self.__require_non_magnetic('batched and adjoint solves')
         return {'chi': self.__chi, 'chi_kind': self.__chi_kind, 's': self.__source_normalized,
                 's_box': self.__source_box,
                 'coef': self.__coef, 'field': self.__fields[self.__cur], 'alpha': self.__alpha, 'alpha_base': self.__alpha_base,

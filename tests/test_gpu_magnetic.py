@@ -1,0 +1,89 @@
+"""Magnetic media (macromax_amd/magnetic.py) on the GPU.
+
+- The reference's magnetic solve (python/tests/test_solution.py:360-418): a plate of refractive index -1.5 (permittivity
+  and permeability both negative) lit at 30 degrees, complex64, residue < 1e-3 within 1000 iterations, and the
+  golden field norms in front of and behind the plate (0.0002582 and 0.0001864, decimal 5), with the derived
+  fields' dtypes.
+- The magnetic iteration (curl terms, its own alpha and beta) converges to the same field as the fused
+  non-magnetic kernels when the medium is non-magnetic up to a bi-anisotropy xi = 1e-7 at one voxel (complex128,
+  both to residue 1e-8; fields within 1e-5 relative).
+"""
+import numpy as np
+import numpy.testing as npt
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_negative_index_plate_matches_reference_norms():
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print('magnetic: negative-index plate', flush=True)
+    wavelength = 500e-9
+    boundary_thickness = 2e-6
+    beam_diameter = 5e-6
+    plate_thickness = 5e-6
+    k0 = 2 * np.pi / wavelength
+    grid = mm.Grid(np.ones(2) * 128, wavelength / 4)
+    angle = 30 * np.pi / 180
+
+    def rot_z(a):
+        return np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    k_inc = rot_z(angle) * k0 @ np.array([1, 0, 0])
+    pol = (rot_z(angle) @ np.array([0, 1, 1j]) / np.sqrt(2))[:, np.newaxis, np.newaxis]
+    J = np.exp(1j * (k_inc[0] * grid[0] + k_inc[1] * grid[1]))
+    src = grid.shape[0] - int(boundary_thickness / grid.step[0])
+    J[:src, :] = 0
+    J[src + 1:, :] = 0
+    J = J * np.exp(-0.5 * ((grid[1] - grid[1].ravel()[grid.shape[0] // 3]) / (beam_diameter / 2)) ** 2)
+    J = J[np.newaxis, ...] * pol
+    n = 1 + (-1.5 - 1) * np.ones(grid[1].shape) * (np.abs(grid[0]) < plate_thickness / 2)
+    bound = LinearBound(grid, thickness=boundary_thickness, max_extinction_coefficient=0.50)
+    sol = mm.solve(grid, vacuum_wavelength=wavelength, current_density=J, refractive_index=n, bound=bound,
+                   callback=lambda s: s.iteration < 1e4 and s.residue > 1e-3, dtype=np.complex64)
+    assert sol.magnetic
+    assert sol.residue < 1e-3, sol.residue
+    assert sol.iteration <= 1000, sol.iteration
+    E = sol.E
+    front = E[:, src + 1:src + 8, :]
+    back = E[:, -src - 8:-src - 1, :]
+    npt.assert_almost_equal(np.linalg.norm(back.ravel()), 0.0001864, decimal=5)
+    npt.assert_almost_equal(np.linalg.norm(front.ravel()), 0.0002582, decimal=5)
+    assert E.dtype == np.complex64
+    assert sol.B.dtype == np.complex64 and sol.D.dtype == np.complex64 and sol.H.dtype == np.complex64
+    assert sol.S.dtype == np.float32
+
+
+@pytest.mark.parametrize('anisotropic', [False, True], ids=['iso', 'aniso'])
+def test_magnetic_iteration_matches_fused_kernels(anisotropic):
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print(f'magnetic vs non-magnetic path {"aniso" if anisotropic else "iso"}', flush=True)
+    wavelength = 500e-9
+    grid = mm.Grid([64, 48], wavelength / 4)
+    rng = np.random.default_rng(4)
+    eps = (1.0 + 0.4 * rng.random(grid.shape)) ** 2
+    if anisotropic:
+        a = rng.standard_normal((3, *grid.shape))
+        a /= np.linalg.norm(a, axis=0)
+        eps = eps * np.eye(3).reshape(3, 3, 1, 1) + 0.3 * a[:, None] * a[None, :]
+    J = np.zeros((3, *grid.shape), dtype=np.complex128)
+    J[1, 12, :] = np.exp(-((np.arange(grid.shape[1]) - 24) / 8.0) ** 2)
+    J[2, 12, :] = 0.5j * J[1, 12, :]
+    bound = LinearBound(grid, thickness=2e-6, max_extinction_coefficient=0.25)
+    xi = np.zeros(grid.shape, dtype=np.complex128)
+    xi[30, 20] = 1e-7  # above the allclose threshold that makes a medium magnetic, physically negligible
+    cb = lambda s: s.iteration < 20000 and s.residue > 1e-8
+    plain = mm.solve(grid, vacuum_wavelength=wavelength, current_density=J, epsilon=eps, bound=bound,
+                     dtype=np.complex128, callback=cb)
+    mag = mm.solve(grid, vacuum_wavelength=wavelength, current_density=J, epsilon=eps, xi=xi, bound=bound,
+                   dtype=np.complex128, callback=cb)
+    assert mag.magnetic and not plain.magnetic
+    assert mag.residue <= 1e-8 and plain.residue <= 1e-8
+    err = np.linalg.norm(mag.E - plain.E) / np.linalg.norm(plain.E)
+    assert err <= 1e-5, err
+    # H of the magnetic path (mu^-1 curl E) equals the non-magnetic one for mu = 1
+    errH = np.linalg.norm(mag.H - plain.H) / np.linalg.norm(plain.H)
+    assert errH <= 1e-5, errH
+    with pytest.raises(NotImplementedError):
+        mag.refine()
