@@ -102,7 +102,10 @@ int mmx_fft(mmx_plan_t plan, const void* in, void* out, int64_t nbatch, int dire
  *     d = chi_eff [ G(chi_eff E + s_eff) - E ],   Eout = E + d,
  * G the scalar (ncomp == 1) or dyadic (ncomp == 3) Green's operator (solver.py:564-584,
  * backend/__init__.py:606-649). chi_kind 1: chi is [1,1,grid] (isotropic); 2: chi is [3,3,grid] (needs ncomp 3);
- * 3: chi is complex-symmetric (chi^T = chi, reciprocal media) packed as [6,grid] = {xx, xy, xz, yy, yz, zz}.
+ * 3: chi is complex-symmetric (chi^T = chi, reciprocal media) packed as [6,grid] = {xx, xy, xz, yy, yz, zz};
+ * 4: chi is uniaxial, chi = u I + w n n^T with a real unit axis n per voxel, stored as [4,grid] complex planes
+ * {u, w, n_x + i n_y, n_z} where the last plane holds n_z as the first grid-size reals (28 bytes per voxel in
+ * complex64 instead of 48; the storage has no reference counterpart, the operator is the reference's chi).
  * E, Eout: [nbatch][ncomp][grid]; s: [nbatch or 1][ncomp][grid] (s_bstride 0 shares one source);
  * chi: [nbatch or 1][chi size] (chi_bstride 0 shares one medium). Eout is also the work buffer; E is only read.
  * norms (device, 2*nbatch doubles): {sum |d|^2, sum |Eout|^2} per problem, accumulated in FP64 with a fixed
@@ -194,7 +197,8 @@ int mmx_norm2(mmx_plan_t plan, const void* x, int64_t n, double* out, void* stre
 
 /*
  * y = chi x per voxel for `nbatch` fields x of ncomp components (BackEnd.mul(chi, x), backend/__init__.py:415-444,
- * torch.py:220-249, for chi of shape [1,1,grid], [3,3,grid] or packed symmetric [6,grid]). y may alias x.
+ * torch.py:220-249, for chi of shape [1,1,grid], [3,3,grid], packed symmetric [6,grid] or uniaxial [4,grid], chi_kind
+ * 1 to 4 as for mmx_born_step). y may alias x.
  */
 int mmx_chi_mul(mmx_plan_t plan, const void* chi, int chi_kind, const void* x, void* y, int ncomp,
                 int64_t nbatch, void* stream);

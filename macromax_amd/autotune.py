@@ -128,7 +128,7 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
     def crand(*shape):
         return torch.complex(torch.randn(*shape, device=dev, generator=g),
                              torch.randn(*shape, device=dev, generator=g)).to(plan.dtype) * 0.1
-    nchi = {1: 1, 2: 9, 3: 6}[chi_kind]
+    nchi = {1: 1, 2: 9, 3: 6, 4: 4}[chi_kind]
     E = crand(nbatch, ncomp, V)
     if operands is not None:
         s_op, chi_op, coef_op, s_box = operands

@@ -146,7 +146,7 @@ class Plan:
             raise ValueError('born_step: norms must be a float64 device tensor of 2*nbatch elements')
         if E.numel() != nbatch * ncomp * self.V or Eout.numel() != E.numel():
             raise ValueError('born_step: E / Eout size mismatch')
-        chi_size = {1: 1, 2: 9, 3: 6}[int(chi_kind)] * self.V
+        chi_size = {1: 1, 2: 9, 3: 6, 4: 4}[int(chi_kind)] * self.V
         if chi.numel() != (chi_size if chi_bstride == 0 else chi_size * nbatch):
             raise ValueError('born_step: chi size mismatch')
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
@@ -168,7 +168,7 @@ class Plan:
         for t, w in others:
             if t.numel() != E.numel():
                 raise ValueError(f'{w} size mismatch')
-        chi_size = {1: 1, 2: 9, 3: 6}[int(chi_kind)] * self.V
+        chi_size = {1: 1, 2: 9, 3: 6, 4: 4}[int(chi_kind)] * self.V
         if chi.numel() != (chi_size if chi_bstride == 0 else chi_size * nbatch):
             raise ValueError('chi size mismatch')
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):

@@ -334,6 +334,7 @@ int launch_reg_kind(int N, bool three, int chi_kind, const TileCfg& t, const Pas
             case 1: return launch_reg_n<KIND, 3, 1>(N, t, a, stream);
             case 2: return launch_reg_n<KIND, 3, 2>(N, t, a, stream);
             case 3: return launch_reg_n<KIND, 3, 3>(N, t, a, stream);
+            case 4: return launch_reg_n<KIND, 3, 4>(N, t, a, stream);
             default: return fail(MMX_EINVAL, "internal: chi kind");
         }
     }
@@ -531,7 +532,7 @@ int reduce_norms(const double* partials, int64_t tiles, int64_t nbatch, double* 
 template <typename T>
 int born_check(const mmx_plan_s* p, int C, int chi_kind) {
     if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
-    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
+    if (chi_kind < 1 || chi_kind > 4) return fail(MMX_EINVAL, "chi_kind must be 1, 2, 3 or 4");
     if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
     if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
     return MMX_OK;
@@ -661,7 +662,7 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
               const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
               hipStream_t stream) {
     if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
-    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
+    if (chi_kind < 1 || chi_kind > 4) return fail(MMX_EINVAL, "chi_kind must be 1, 2, 3 or 4");
     if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
     if (nbatch <= 0) return MMX_OK;
     if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
@@ -792,7 +793,7 @@ template <typename T>
 int chi_mul_impl(mmx_plan_s* p, const void* chi, int chi_kind, const void* x, void* y, int C, int64_t nbatch,
                  hipStream_t stream) {
     if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
-    if (chi_kind < 1 || chi_kind > 3) return fail(MMX_EINVAL, "chi_kind must be 1, 2 or 3");
+    if (chi_kind < 1 || chi_kind > 4) return fail(MMX_EINVAL, "chi_kind must be 1, 2, 3 or 4");
     hipLaunchKernelGGL((chi_mul_kernel<T>), dim3(grid_for(p->V * nbatch)), dim3(256), 0, stream,
                        reinterpret_cast<const cx<T>*>(chi), chi_kind, reinterpret_cast<const cx<T>*>(x),
                        reinterpret_cast<cx<T>*>(y), C, p->V, nbatch);

@@ -152,7 +152,14 @@ __device__ __forceinline__ void unravel3(int64_t v, const PassArgs<T>& a, int64_
 }
 
 // chi at one voxel, loaded once and applied as often as needed. chi_kind 1: one value per voxel; 2: 3x3
-// row-major [9, V]; 3: complex-symmetric 3x3 packed as [xx, xy, xz, yy, yz, zz] ([6, V]).
+// row-major [9, V]; 3: complex-symmetric 3x3 packed as [xx, xy, xz, yy, yz, zz] ([6, V]); 4: uniaxial
+// chi = u I + w n n^T with a real unit axis n, stored as planes [u, w, (n_x, n_y), n_z] ([4, V] complex with the
+// real n_z in the first V reals of the last plane): 28 bytes per voxel in complex64 instead of 48.
+template <typename T>
+__device__ __forceinline__ T uniaxial_nz(const cx<T>* chi_b, int64_t V, int64_t v) {
+    return reinterpret_cast<const T*>(chi_b + 3 * V)[v];
+}
+
 template <typename T>
 struct ChiVox {
     cx<T> m0, m1, m2, m3, m4, m5, m6, m7, m8;  // named (not an array): every access stays in registers
@@ -162,6 +169,12 @@ template <typename T>
 __device__ __forceinline__ void chi_load(const PassArgs<T>& a, const cx<T>* chi_b, int64_t v, ChiVox<T>& q) {
     const int64_t V = a.comp_stride;
     q.m0 = chi_b[v];
+    if (a.chi_kind == 4) {
+        q.m1 = chi_b[V + v];
+        q.m2 = chi_b[2 * V + v];
+        q.m3 = cx<T>{uniaxial_nz(chi_b, V, v), T(0)};
+        return;
+    }
     if (a.chi_kind >= 2) {
         q.m1 = chi_b[V + v];
         q.m2 = chi_b[2 * V + v];
@@ -187,7 +200,12 @@ __device__ __forceinline__ void chi_mulv(const PassArgs<T>& a, const ChiVox<T>& 
         return;
     }
     cx<T> y0, y1, y2;
-    if (a.chi_kind == 2) {
+    if (a.chi_kind == 4) {  // u x + w n (n . x)
+        const cx<T> t = q.m1 * (scal(x[0], q.m2.r) + scal(x[1], q.m2.i) + scal(x[2], q.m3.r));
+        y0 = q.m0 * x[0] + scal(t, q.m2.r);
+        y1 = q.m0 * x[1] + scal(t, q.m2.i);
+        y2 = q.m0 * x[2] + scal(t, q.m3.r);
+    } else if (a.chi_kind == 2) {
         y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
         y1 = q.m3 * x[0] + q.m4 * x[1] + q.m5 * x[2];
         y2 = q.m6 * x[0] + q.m7 * x[1] + q.m8 * x[2];
@@ -584,7 +602,8 @@ __global__ void __launch_bounds__(256) sum_kernel(const double* partials, int64_
     if (threadIdx.x == 0) out[0] = s;
 }
 
-// y = chi x per voxel (chi [1,1,V], [3,3,V] or packed symmetric [6,V]; x, y [C,V] with batch strides).
+// y = chi x per voxel (chi [1,1,V], [3,3,V], packed symmetric [6,V] or uniaxial [4,V]; x, y [C,V] with batch
+// strides).
 template <typename T>
 __global__ void __launch_bounds__(256) chi_mul_kernel(const cx<T>* chi, int chi_kind, const cx<T>* x, cx<T>* y,
                                                       int C, int64_t V, int64_t nbatch) {
@@ -597,6 +616,14 @@ __global__ void __launch_bounds__(256) chi_mul_kernel(const cx<T>* chi, int chi_
         if (chi_kind == 1) {
             const cx<T> c0 = chi[v];
             for (int c = 0; c < C; ++c) yb[c * V + v] = c0 * xb[c * V + v];
+        } else if (chi_kind == 4) {
+            const cx<T> u = chi[v], w = chi[V + v], nxy = chi[2 * V + v];
+            const T nz = uniaxial_nz(chi, V, v);
+            const cx<T> x0 = xb[v], x1 = xb[V + v], x2 = xb[2 * V + v];
+            const cx<T> t = w * (scal(x0, nxy.r) + scal(x1, nxy.i) + scal(x2, nz));
+            yb[v] = u * x0 + scal(t, nxy.r);
+            yb[V + v] = u * x1 + scal(t, nxy.i);
+            yb[2 * V + v] = u * x2 + scal(t, nz);
         } else {
             const cx<T> xv[3] = {xb[v], xb[V + v], xb[2 * V + v]};
             cx<T> m[9];

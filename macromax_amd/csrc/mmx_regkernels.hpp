@@ -19,10 +19,21 @@ template <typename T> __device__ __forceinline__ cx<T>& at(cx<T>* base, uint32_t
     return *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(base) + boff);
 }
 
-// chi of one voxel for a compile-time chi kind CK (1: scalar, 2: full 3x3 row-major, 3: packed symmetric).
+// chi of one voxel for a compile-time chi kind CK (1: scalar, 2: full 3x3 row-major, 3: packed symmetric,
+// 4: uniaxial u I + w n n^T).
+// chi_t is the chi of the tile (chi + tile offset tb, in complex elements), boff the byte offset of the voxel from it.
 template <typename T, int CK>
-__device__ __forceinline__ void chi_load_ck(const cx<T>* chi_t, int64_t V, uint32_t boff, ChiVox<T>& q) {
+__device__ __forceinline__ void chi_load_ck(const cx<T>* chi_t, int64_t V, uint32_t boff, ChiVox<T>& q, int64_t tb) {
     q.m0 = at(chi_t, boff);
+    if constexpr (CK == 4) {
+        q.m1 = at(chi_t + V, boff);
+        q.m2 = at(chi_t + 2 * V, boff);
+        // n_z: the reals of the last plane, element tb + boff / sizeof(cx) of that real array
+        const T* nz_t = reinterpret_cast<const T*>(chi_t - tb + 3 * V) + tb;
+        q.m3.r = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(nz_t) + boff / 2);
+        q.m3.i = T(0);
+        return;
+    }
     if constexpr (CK >= 2) {
         q.m1 = at(chi_t + V, boff);
         q.m2 = at(chi_t + 2 * V, boff);
@@ -46,7 +57,12 @@ __device__ __forceinline__ void chi_mul_ck(const ChiVox<T>& q, const cx<T>* x, c
         for (int c = 0; c < NC; ++c) y[c] = scal(q.m0 * x[c], cf.a) + cc * x[c];
     } else {
         cx<T> y0, y1, y2;
-        if constexpr (CK == 2) {
+        if constexpr (CK == 4) {  // u x + w n (n . x)
+            const cx<T> t = q.m1 * (scal(x[0], q.m2.r) + scal(x[1], q.m2.i) + scal(x[2], q.m3.r));
+            y0 = q.m0 * x[0] + scal(t, q.m2.r);
+            y1 = q.m0 * x[1] + scal(t, q.m2.i);
+            y2 = q.m0 * x[2] + scal(t, q.m3.r);
+        } else if constexpr (CK == 2) {
             y0 = q.m0 * x[0] + q.m1 * x[1] + q.m2 * x[2];
             y1 = q.m3 * x[0] + q.m4 * x[1] + q.m5 * x[2];
             y2 = q.m6 * x[0] + q.m7 * x[1] + q.m8 * x[2];
@@ -70,10 +86,10 @@ template <typename T, int NC, bool WITH_S> struct EpiIn {
 
 template <typename T, int NC, int CK, bool WITH_S>
 __device__ __forceinline__ void epi_load(EpiIn<T, NC, WITH_S>& in, const cx<T>* Et, const cx<T>* chit,
-                                         const cx<T>* st, int64_t V, uint32_t bo) {
+                                         const cx<T>* st, int64_t V, uint32_t bo, int64_t tb) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) in.e[c] = at(Et + c * V, bo);
-    chi_load_ck<T, CK>(chit, V, bo, in.q);
+    chi_load_ck<T, CK>(chit, V, bo, in.q, tb);
     if constexpr (WITH_S) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) in.s[c] = at(st + c * V, bo);
@@ -171,7 +187,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) x[c] = at(Et + c * V, bo);
                 ChiVox<T> qv;
-                chi_load_ck<T, CK>(chit, V, bo, qv);
+                chi_load_ck<T, CK>(chit, V, bo, qv, tb);
                 chi_mul_ck<T, CK, NC>(qv, x, y, cf);
                 const int j = t + m * TP;
                 if (s_row && j >= s_jlo && j < s_jhi) {
@@ -242,10 +258,10 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
             // for the fused next prologue, s) is requested before voxel m's result is stored.
             constexpr bool S_INLINE = F::PRO_AFTER && SB == 0;
             EpiIn<T, NC, S_INLINE> cur, nxt;
-            epi_load<T, NC, CK, S_INLINE>(cur, Et, chit, st, V, toff);
+            epi_load<T, NC, CK, S_INLINE>(cur, Et, chit, st, V, toff, tb);
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                if (m + 1 < E) epi_load<T, NC, CK, S_INLINE>(nxt, Et, chit, st, V, toff + (m + 1) * mstep);
+                if (m + 1 < E) epi_load<T, NC, CK, S_INLINE>(nxt, Et, chit, st, V, toff + (m + 1) * mstep, tb);
                 const uint32_t bo = toff + m * mstep;
                 cx<T> w[3] = {zero, zero, zero}, d[3] = {zero, zero, zero}, en[3] = {zero, zero, zero};
 #pragma unroll

@@ -219,10 +219,23 @@ class Solution:
             chi = chi - torch.eye(3, dtype=torch.complex128, device=BE.device).reshape(3, 3, *([1] * self.grid.ndim)) \
                 * (alpha * 1.0j / alpha.imag)
         chi = chi.expand(chi.shape[0], chi.shape[1], *full)
+        # reciprocal media (eps^T = eps to 8 rounding units of the element type) are stored symmetrically
+        symmetric = False
+        uniaxial = None
+        if chi.shape[0] == 3:
+            asym = torch.amax(torch.abs(eps64 - eps64.transpose(0, 1)))
+            symmetric = bool(asym <= 8 * torch.finfo(BE.hardware_dtype).eps * torch.amax(torch.abs(eps64)))
+            if symmetric:
+                chi = 0.5 * (chi + chi.transpose(0, 1))
+                uniaxial = _uniaxial_planes(chi, BE.hardware_dtype)
         if chi.shape[0] == 1:
             self.__chi_kind = 1
             self.__chi = chi.to(BE.hardware_dtype).contiguous()
-        elif torch.equal(eps64, eps64.transpose(0, 1)):
+        elif uniaxial is not None:
+            # uniaxial medium: chi = u I + w n n^T, stored as [u, w, (n_x, n_y), n_z] (28 bytes per voxel in c64)
+            self.__chi_kind = 4
+            self.__chi = uniaxial
+        elif symmetric:
             # reciprocal medium: chi^T = chi, stored as the 6 distinct entries {xx, xy, xz, yy, yz, zz}
             self.__chi_kind = 3
             self.__chi = torch.stack([chi[0, 0], chi[0, 1], chi[0, 2], chi[1, 1], chi[1, 2], chi[2, 2]]).to(
@@ -342,13 +355,16 @@ class Solution:
 
     @property
     def chi_kind(self) -> int:
-        """Storage of chi on the device: 1 isotropic, 2 full 3x3, 3 complex-symmetric 3x3 (6 entries)."""
+        """Storage of chi on the device: 1 isotropic, 2 full 3x3, 3 complex-symmetric 3x3 (6 entries), 4 uniaxial
+        (u I + w n n^T with a real unit axis n)."""
         return self.__chi_kind
 
     @property
     def chi(self) -> torch.Tensor:
         """The susceptibility chi for the alpha chosen at set-up, as a [1,1,*grid] or [3,3,*grid] tensor."""
         self.__require_non_magnetic('chi')
+        if self.__chi_kind == 4:
+            return _uniaxial_matrix(self.__chi)
         if self.__chi_kind != 3:
             return self.__chi
         c = self.__chi
@@ -782,6 +798,59 @@ class Solution:
             if not callback(sol):
                 break
         return self
+
+
+def _uniaxial_planes(chi: torch.Tensor, dtype) -> torch.Tensor | None:
+    """chi [3, 3, *grid] (complex128, symmetric) as chi = u I + w n n^T with a real unit axis n per voxel, stored as
+    [4, *grid] planes [u, w, n_x + i n_y, n_z (in the first reals of the plane)] of ``dtype``; None when chi is not of
+    that form to within 8 rounding units of ``dtype`` (relative to its largest entry), about the rounding of storing
+    chi itself in ``dtype``.
+
+    With A = chi - tr(chi)/3 I = w (n n^T - I/3), A A^H = |w|^2 (n n^T / 3 + I / 9) is real, a n n^T + b I: b follows
+    from its trace and the trace of its square, n from the largest column of A A^H - b I, and
+    w = 3/2 (n^T chi n - tr(chi) / 3), u = (tr(chi) - w) / 3."""
+    shape = chi.shape[2:]
+    eye = torch.eye(3, dtype=chi.dtype, device=chi.device).reshape(3, 3, *([1] * len(shape)))
+    tr = chi[0, 0] + chi[1, 1] + chi[2, 2]
+    A = chi - eye * (tr / 3)
+    P = torch.einsum('ij...,kj...->ik...', A, A.conj()).real
+    T1 = P[0, 0] + P[1, 1] + P[2, 2]
+    T2 = torch.einsum('ij...,ji...->...', P, P)
+    b = (4 * T1 - torch.sqrt(torch.clamp(24 * T2 - 8 * T1 ** 2, min=0))) / 12
+    Q = P - torch.eye(3, dtype=P.dtype, device=P.device).reshape(3, 3, *([1] * len(shape))) * b
+    diag = torch.stack([Q[0, 0], Q[1, 1], Q[2, 2]])
+    j = torch.argmax(diag, dim=0)
+    col = torch.gather(Q, 1, j.reshape(1, 1, *shape).expand(3, 1, *shape))[:, 0]  # column j of Q
+    cj = torch.gather(diag, 0, j.unsqueeze(0))[0]
+    nrm = torch.sqrt(torch.clamp(torch.sum(col ** 2, dim=0), min=0))
+    scale = torch.amax(torch.abs(chi))
+    flat = cj <= 1e-24 * scale ** 2  # isotropic voxel: any axis
+    n = torch.where(flat.unsqueeze(0), torch.tensor([0.0, 0.0, 1.0], dtype=P.dtype, device=P.device).reshape(
+        3, *([1] * len(shape))), col / torch.where(flat, torch.ones_like(nrm), nrm).unsqueeze(0))
+    nc = n.to(chi.dtype)
+    ntchin = torch.einsum('i...,ij...,j...->...', nc, chi, nc)
+    w = 1.5 * (ntchin - tr / 3)
+    u = (tr - w) / 3
+    recon = eye * u + w * nc.unsqueeze(0) * nc.unsqueeze(1)
+    if not bool(torch.amax(torch.abs(recon - chi)) <= 8 * torch.finfo(dtype).eps * scale):
+        return None
+    real = torch.float32 if dtype == torch.complex64 else torch.float64
+    out = torch.zeros((4, *shape), dtype=dtype, device=chi.device)
+    out[0] = u.to(dtype)
+    out[1] = w.to(dtype)
+    out[2] = torch.complex(n[0], n[1]).to(dtype)
+    out[3].view(real).reshape(-1)[:int(np.prod(shape))] = n[2].reshape(-1).to(real)
+    return out
+
+
+def _uniaxial_matrix(planes: torch.Tensor) -> torch.Tensor:
+    """[3, 3, *grid] chi from the uniaxial planes of ``_uniaxial_planes``."""
+    shape = planes.shape[1:]
+    real = torch.float32 if planes.dtype == torch.complex64 else torch.float64
+    nz = planes[3].view(real).reshape(-1)[:int(np.prod(shape))].reshape(shape)
+    n = torch.stack([planes[2].real, planes[2].imag, nz]).to(planes.dtype)
+    eye = torch.eye(3, dtype=planes.dtype, device=planes.device).reshape(3, 3, *([1] * len(shape)))
+    return eye * planes[0] + planes[1] * n.unsqueeze(0) * n.unsqueeze(1)
 
 
 def source_support(s: torch.Tensor, ndim: int):

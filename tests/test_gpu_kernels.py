@@ -61,6 +61,11 @@ def _born_case(shape, C, kind, nbatch=1, seed=0, chi_shared=True):
     nch = 1 if chi_shared else nbatch
     if kind == 1:
         chi = 0.3 * crand(nch, 1, 1, *shape)
+    elif kind == 4:  # uniaxial: u I + w n n^T with a real unit axis n
+        n = rng.standard_normal((nch, 3, *shape))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        eye = np.eye(3).reshape(1, 3, 3, *([1] * len(shape)))
+        chi = 0.3 * crand(nch, 1, 1, *shape) * eye + 0.3 * crand(nch, 1, 1, *shape) * n[:, :, None] * n[:, None, :]
     else:
         chi = 0.2 * crand(nch, 3, 3, *shape)
         if kind == 3:
@@ -71,10 +76,25 @@ def _born_case(shape, C, kind, nbatch=1, seed=0, chi_shared=True):
     return E, s, chi, alphas, a, c
 
 
-def _pack(chi, kind):
+def _pack(chi, kind, dtype=np.complex128):
+    """Device storage of chi: [1,1,*g], [3,3,*g], packed symmetric [6,*g] or uniaxial [u, w, n_x + i n_y, n_z]
+    (n_z in the first reals of the last plane), the latter recovered here with numpy.linalg.eigh."""
+    if kind == 4:
+        shape = chi.shape[2:]
+        m = np.moveaxis(chi.reshape(3, 3, -1), -1, 0)  # [V, 3, 3]
+        tr = np.trace(m, axis1=1, axis2=2)
+        A = m - tr[:, None, None] / 3 * np.eye(3)
+        _, vec = np.linalg.eigh((A @ np.conj(np.swapaxes(A, 1, 2))).real)
+        n = vec[:, :, 2]
+        w = 1.5 * (np.einsum('vi,vij,vj->v', n, m, n) - tr / 3)
+        u = (tr - w) / 3
+        out = np.zeros((4, n.shape[0]), dtype=dtype)
+        out[0], out[1], out[2] = u, w, n[:, 0] + 1j * n[:, 1]
+        out[3].view(np.float32 if dtype == np.complex64 else np.float64)[:n.shape[0]] = n[:, 2]
+        return out.reshape(4, *shape)
     if kind != 3:
-        return chi
-    return np.stack([chi[0, 0], chi[0, 1], chi[0, 2], chi[1, 1], chi[1, 2], chi[2, 2]])
+        return chi.astype(dtype)
+    return np.stack([chi[0, 0], chi[0, 1], chi[0, 2], chi[1, 1], chi[1, 2], chi[2, 2]]).astype(dtype)
 
 
 def _oracle_update(E, s, chi, alpha, a, c, k, vectorial):
@@ -93,6 +113,8 @@ BORN_CASES = [
     ((16, 20, 24), 3, 3), ((64, 48), 3, 3), ((512,), 3, 3),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
+    # uniaxial chi storage, LDS-staged and register kernels
+    ((16, 20, 24), 3, 4), ((64, 48), 3, 4), ((128, 64, 256), 3, 4), ((128, 256), 3, 4),
 ]
 
 
@@ -113,7 +135,8 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
     Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
     Eo = torch.zeros_like(Et)
     st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
-    ct = torch.as_tensor(_pack(chi[0], kind), dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64 if tdt == torch.complex64 else np.complex128),
+                         device=device).contiguous()
     coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
     norms = torch.zeros(2, dtype=torch.float64, device=device)
     plan.born_step(Et, Eo, st, ct, C, kind, coef, 1, 0, 0, norms)
@@ -131,7 +154,8 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
 
 @pytest.mark.parametrize('shape,C,kind,shared', [((64, 48), 1, 1, True), ((12, 16, 20), 3, 2, True),
                                                  ((12, 16, 20), 3, 1, False), ((512,), 3, 2, False),
-                                                 ((12, 16, 20), 3, 3, False)],
+                                                 ((12, 16, 20), 3, 3, False), ((12, 16, 20), 3, 4, False),
+                                                 ((64, 32, 64), 3, 4, False)],
                          ids=lambda v: str(v))
 def test_born_step_batched(shape, C, kind, shared, device):
     import torch
@@ -144,12 +168,12 @@ def test_born_step_batched(shape, C, kind, shared, device):
     Et = torch.as_tensor(E, dtype=torch.complex64, device=device).contiguous()
     Eo = torch.zeros_like(Et)
     st = torch.as_tensor(s, dtype=torch.complex64, device=device).contiguous()
-    ct = torch.as_tensor(np.stack([_pack(_, kind) for _ in chi]), dtype=torch.complex64, device=device).contiguous()
+    ct = torch.as_tensor(np.stack([_pack(_, kind, np.complex64) for _ in chi]), device=device).contiguous()
     coef = torch.tensor([[al.real, al.imag, aa, cc.real, cc.imag, 0.0] for al, aa, cc in zip(alphas, a, c)],
                         dtype=torch.float32, device=device)
     norms = torch.zeros(2 * nb, dtype=torch.float64, device=device)
     V = int(np.prod(shape))
-    chi_size = {1: 1, 2: 9, 3: 6}[kind] * V
+    chi_size = {1: 1, 2: 9, 3: 6, 4: 4}[kind] * V
     plan.born_step(Et, Eo, st, ct, C, kind, coef, nb, C * V, 0 if shared else chi_size, norms)
     torch.cuda.synchronize()
     out = Eo.cpu().numpy()
@@ -188,7 +212,9 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
                                                 ((1024,), 3, 2, 'c64'), ((512,), 3, 3, 'c64'),
                                                 ((64, 128, 64), 3, 1, 'c64'), ((128, 64, 256), 3, 3, 'c64'),
                                                 ((64, 64, 128), 1, 1, 'c64'), ((256, 64), 3, 2, 'c64'),
-                                                ((16, 20, 24), 3, 3, 'c128'), ((48, 64), 1, 1, 'c128')],
+                                                ((16, 20, 24), 3, 3, 'c128'), ((48, 64), 1, 1, 'c128'),
+                                                ((256, 256, 256), 3, 4, 'c64'), ((64, 128, 64), 3, 4, 'c64'),
+                                                ((16, 20, 24), 3, 4, 'c128')],
                          ids=lambda v: str(v))
 def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
@@ -202,7 +228,8 @@ def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     plan = _plan(shape, step, tdt, device)
     Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
     st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
-    ct = torch.as_tensor(_pack(chi[0], kind), dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64 if tdt == torch.complex64 else np.complex128),
+                         device=device).contiguous()
     coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
     n1 = torch.zeros(2, dtype=torch.float64, device=device)
     n2 = torch.zeros(2, dtype=torch.float64, device=device)
@@ -222,7 +249,8 @@ def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
 
 
 @pytest.mark.parametrize('shape,C,kind', [((64, 128, 64), 3, 3), ((20, 24, 40), 3, 2), ((48, 40), 1, 1),
-                                          ((256,), 3, 1), ((256, 64), 3, 1)], ids=lambda v: str(v))
+                                          ((256,), 3, 1), ((256, 64), 3, 1), ((64, 128, 64), 3, 4)],
+                         ids=lambda v: str(v))
 def test_source_box(shape, C, kind, device):
     """A source zero outside a box: the box-restricted passes (born_step, prologue/iterate) give the same update as
     the dense ones; a box that cuts the source off changes it (the box is honoured)."""
@@ -239,7 +267,7 @@ def test_source_box(shape, C, kind, device):
     plan = _plan(shape, step, torch.complex64, device)
     Et = torch.as_tensor(E[0], dtype=torch.complex64, device=device).contiguous()
     st = torch.as_tensor(s0, dtype=torch.complex64, device=device).contiguous()
-    ct = torch.as_tensor(_pack(chi[0], kind), dtype=torch.complex64, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64), device=device).contiguous()
     coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=torch.float32,
                         device=device)
     out = {}

@@ -273,3 +273,33 @@ def test_derived_fields_of_plane_wave():
     npt.assert_allclose(pairs(divS.real), dS, rtol=0, atol=5e-2 * np.max(np.abs(dS)))
     assert np.max(pairs(divS.real)) < 0  # absorption
     assert np.max(np.abs(divS[sel])) <= 5e-2 * np.max(np.abs(S[0, sel]))  # no sources or losses in vacuum
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128], ids=['c64', 'c128'])
+def test_uniaxial_storage(dtype):
+    """Uniaxial media (eps = a I + b n n^T per voxel, as grains of a birefringent crystal) are stored as
+    [u, w, n] (chi kind 4); a biaxial medium keeps the packed symmetric storage (kind 3). The stored operator equals
+    chi = (i/alpha_i)(eps/beta - alpha I) to 8 rounding units of the element type."""
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print(f'uniaxial storage {np.dtype(dtype).name}', flush=True)
+    rng = np.random.default_rng(7)
+    grid = mm.Grid([16, 20, 12], 0.1e-6)
+    a = rng.standard_normal((3, *grid.shape))
+    a /= np.linalg.norm(a, axis=0)
+    eps = 1.5 ** 2 * np.eye(3).reshape(3, 3, 1, 1, 1) + (1.65 ** 2 - 1.5 ** 2) * a[:, None] * a[None, :]
+    J = np.zeros((3, *grid.shape), dtype=dtype)
+    J[0, :, :, 3] = 1
+    bound = LinearBound(grid, thickness=0.4e-6)
+    sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, current_density=J, bound=bound, dtype=dtype)
+    assert sol.chi_kind == 4
+    al, beta = sol.alpha, sol.beta
+    e = eps + np.eye(3).reshape(3, 3, 1, 1, 1) * bound.electric_susceptibility
+    ref = (e / beta - al * np.eye(3).reshape(3, 3, 1, 1, 1)) * (1j / al.imag)
+    chi = sol.chi.cpu().numpy()
+    assert np.max(np.abs(chi - ref)) <= 8 * np.finfo(dtype).eps * np.max(np.abs(ref))
+    eps_b = eps.copy()
+    eps_b[0, 0] += 0.2  # biaxial
+    sol_b = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps_b, current_density=J, bound=bound, dtype=dtype)
+    assert sol_b.chi_kind == 3
