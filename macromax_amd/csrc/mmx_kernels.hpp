@@ -114,19 +114,17 @@ __device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, b
     // i alpha_i / alpha / N = alpha_i (alpha_i + i alpha_r) / |alpha|^2 / N
     const T rl = cf.alpha_i * a.inv_ntotal / (cf.alpha_r * cf.alpha_r + cf.alpha_i * cf.alpha_i);
     const cx<T> gl{cf.alpha_i * rl, cf.alpha_r * rl};
-    cx<T> pl[3];
-    if (dc) {
+    if (dc) {  // k = 0: the first nd_data components count as longitudinal
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pl[c] = (c < a.nd_data) ? f[c] : cx<T>{T(0), T(0)};
-    } else {
-        cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
-        proj = scal(proj, recip(kk));
-        pl[0] = scal(proj, k0);
-        pl[1] = scal(proj, k1);
-        pl[2] = scal(proj, k2);
+        for (int c = 0; c < 3; ++c) f[c] = (c < a.nd_data ? gl : g) * f[c];
+        return;
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[c] = g * (f[c] - pl[c]) + gl * pl[c];
+    // g (F - P_L F) + gl P_L F = g F + (gl - g) k (k . F) / k^2
+    const cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
+    const cx<T> q = (gl - g) * scal(proj, recip(kk));
+    f[0] = g * f[0] + scal(q, k0);
+    f[1] = g * f[1] + scal(q, k1);
+    f[2] = g * f[2] + scal(q, k2);
 }
 
 // Is the pencil with outer index `row` (the axes before the pass axis, C order, one component of one problem)

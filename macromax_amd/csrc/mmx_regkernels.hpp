@@ -215,24 +215,32 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         }
     }
 
-    if constexpr (F::FWD_FIRST) reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    // k components for the Green pass, loaded before the forward FFT so that the loads complete during it: the
+    // components off the pass axis are fixed for this thread's pencil, the one along it varies with the point
+    T kf[3], kpre[F::GREEN ? E : 1];
+    bool off_dc = true;
     if constexpr (F::GREEN) {
-        // k components off the pass axis are fixed for this thread's pencil
-        int64_t col = rows ? 0 : ib + p;
-        int64_t row = rows ? r0 + p : o;
+        // pencils past the edge of a partial tile use index 0 (their results are not stored)
+        int64_t col = (rows || !valid) ? 0 : ib + p;
+        int64_t row = !valid ? 0 : (rows ? r0 + p : o);
         int64_t i0 = 0, i1 = 0, i2 = 0;
         if (a.axis_orig == 0) { i1 = fdiv64(col, a.fn2); i2 = col - i1 * a.n2; }
         else if (a.axis_orig == 1) { i0 = row; i2 = col; }
         else { i0 = fdiv64(row, a.fn1); i1 = row - i0 * a.n1; }
-        T kf[3];
         kf[0] = (a.axis_orig != 0) ? a.kvec[i0] : T(0);
         kf[1] = (a.axis_orig != 1 && a.n1 > 1) ? a.kvec[a.k1ofs + i1] : T(0);
         kf[2] = (a.axis_orig != 2 && a.n2 > 1) ? a.kvec[a.k2ofs + i2] : T(0);
-        const bool off_dc = (i0 | i1 | i2) != 0;
+        off_dc = (i0 | i1 | i2) != 0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) kpre[m] = a.kvec[a.kofs_axis + t + m * TP];
+    }
+
+    if constexpr (F::FWD_FIRST) reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    if constexpr (F::GREEN) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int j = t + m * TP;
-            const T kj = a.kvec[a.kofs_axis + j];
+            const T kj = kpre[m];
             const T k0 = a.axis_orig == 0 ? kj : kf[0];
             const T k1 = a.axis_orig == 1 ? kj : kf[1];
             const T k2 = a.axis_orig == 2 ? kj : kf[2];
