@@ -252,6 +252,15 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
             for (int c = 0; c < NC; ++c) v[c][m] = f[c];
         }
     }
+    // the update's inputs for the first voxel are requested before the inverse FFT, so that they arrive during it
+    constexpr bool S_INLINE = F::PRO_AFTER && SB == 0;
+    EpiIn<T, NC, S_INLINE> cur;
+    if constexpr (F::EPI) {
+        if (valid) {
+            epi_load<T, NC, CK, S_INLINE>(cur, Ecur + b * a.E_bstride + tb, a.chi + b * a.chi_bstride + tb,
+                                          a.s + b * a.s_bstride + tb, a.comp_stride, toff, tb);
+        }
+    }
     if constexpr (F::INV) reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
 
     if constexpr (F::EPI) {
@@ -263,10 +272,9 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         double dd = 0.0, ee = 0.0;
         if (valid) {
             // Loads and stores retire in issue order on this hardware, so every input of voxel m + 1 (E, chi and,
-            // for the fused next prologue, s) is requested before voxel m's result is stored.
-            constexpr bool S_INLINE = F::PRO_AFTER && SB == 0;
-            EpiIn<T, NC, S_INLINE> cur, nxt;
-            epi_load<T, NC, CK, S_INLINE>(cur, Et, chit, st, V, toff, tb);
+            // for the fused next prologue, s) is requested before voxel m's result is stored (two voxels ahead
+            // measured no faster).
+            EpiIn<T, NC, S_INLINE> nxt;
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 if (m + 1 < E) epi_load<T, NC, CK, S_INLINE>(nxt, Et, chit, st, V, toff + (m + 1) * mstep, tb);

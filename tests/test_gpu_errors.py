@@ -45,7 +45,7 @@ def test_born_call_argument_checks(device):
     L = _native.lib()
     st = torch.cuda.current_stream(device).cuda_stream
     before = Eo.clone()
-    for ncomp, kind, out in ((3, 4, Eo), (2, 1, Eo), (3, 3, E)):
+    for ncomp, kind, out in ((3, 5, Eo), (3, 0, Eo), (2, 1, Eo), (3, 3, E)):
         rc = L.mmx_born_step(plan.handle, E.data_ptr(), out.data_ptr(), s.data_ptr(), chi.data_ptr(), ncomp, kind,
                              coef.data_ptr(), 1, 0, 0, norms.data_ptr(), st)
         assert rc == -1, (ncomp, kind)
