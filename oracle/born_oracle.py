@@ -30,6 +30,10 @@ oracle function                         reference
                                         autograd through ``oracle/born_torch.py``
 ``stockham_fft``                        model of the GPU kernels' Stockham pass schedule (no reference
                                         counterpart: the reference calls numpy.fft / torch.fft)
+``curl``                                ``backend/__init__.py:472-509`` (curl_ft, i k x F, through the FFT)
+``magnetic_setup``                      ``solver.py:350-493`` (mu^-1, chi_EH, chi_HE, eps - xi mu^-1 zeta, the
+                                        two-parameter fmin over (alpha_r, sqrt(beta)), chi_EE/EH/HE/HH)
+``MagneticBornOracle``                  ``solver.py:496-552, 1064-1098`` (curl-term chi operator, iteration)
 =====================================  =====================================================================
 
 Pinning (what checks each function against something other than its own formula) is listed in
@@ -45,7 +49,7 @@ __all__ = [
     'k_vectors', 'linear_bound_extinction', 'bound_susceptibility', 'sigma_ee', 'find_alpha_beta',
     'increase_bias', 'chi_base', 'normalized_source', 'longitudinal_projection_ft', 'green_op',
     'chi_mul', 'BornOracle', 'stockham_fft', 'plan_radices', 'mat3_max_eig_hermitian', 'setup_problem',
-    'solve_to', 'adjoint_gradient', 'transposed_k_vectors',
+    'solve_to', 'adjoint_gradient', 'transposed_k_vectors', 'curl', 'magnetic_setup', 'MagneticBornOracle',
 ]
 
 
@@ -337,6 +341,164 @@ def adjoint_gradient(chi, alpha, beta, k, vectorial, E, g, tol=1e-12, max_iter=2
     else:
         grad = np.conj(mu[:, None, 0] * E[None, :, 0]) / beta  # [3,3,*shape]
     return grad, o
+
+
+# ----------------------------------------------------------------------------------------------------------
+# Magnetic and bi-(an)isotropic media
+# ----------------------------------------------------------------------------------------------------------
+def curl(F, k):
+    """curl F = ift(i k x ft(F)) of a [3,1,*shape] field, k in units of k0 (``backend/__init__.py:472-509``)."""
+    shape = F.shape[2:]
+    axes = tuple(range(2, F.ndim))
+    G = np.fft.fftn(F, axes=axes)
+    kb = [_kb(k[a], a, len(shape)) if a < len(shape) else 0.0 for a in range(3)]
+    out = np.zeros((3, 1, *shape), dtype=np.complex128)
+    out[0, 0] = 1j * (kb[1] * G[2, 0] - kb[2] * G[1, 0])
+    out[1, 0] = 1j * (kb[2] * G[0, 0] - kb[0] * G[2, 0])
+    out[2, 0] = 1j * (kb[0] * G[1, 0] - kb[1] * G[0, 0])
+    return np.fft.ifftn(out, axes=axes)
+
+
+def _matrix_field(a, ndim):
+    a = np.asarray(a, dtype=np.complex128)
+    while a.ndim < ndim + 2:
+        a = a[np.newaxis]
+    return a
+
+
+def _as3(a):
+    return a if a.shape[0] == 3 else np.eye(3).reshape(3, 3, *([1] * (a.ndim - 2))) * a
+
+
+def _mmul(a, b):
+    if a.shape[0] == 1 and b.shape[0] == 1:
+        return a * b
+    return np.einsum('ij...,jk...->ik...', _as3(a), _as3(b))
+
+
+def _max_eig_herm(h):
+    """Largest |eigenvalue| over voxels of a Hermitian [n,n,*shape] field (numpy.linalg.eigvalsh)."""
+    if h.shape[0] == 1:
+        return float(np.amax(np.abs(h)))
+    m = np.moveaxis(h.reshape(3, 3, -1), -1, 0)
+    return float(np.amax(np.abs(np.linalg.eigvalsh(m))))
+
+
+def _adj(a):
+    return np.conj(np.swapaxes(a, 0, 1))
+
+
+def magnetic_setup(shape, pitch, wavenumber, eps, xi, zeta, mu, bound_thickness=(), alpha_beta=None):
+    """alpha, beta and (chi_EE, chi_EH, chi_HE, chi_HH) of a magnetic medium (``solver.py:350-493``).
+
+    mu^-1 per voxel; chi_EH,beta = -i xi mu^-1, chi_HE,beta = i mu^-1 zeta; eps' = eps - xi mu^-1 zeta;
+    sigma_D = |pi / pitch| / k0; sigma_HH(beta) = sqrt(lambda_max((mu^-1 - beta)^H (mu^-1 - beta))) / beta;
+    sigma_EE(alpha_r, beta) = sqrt(lambda_max((eps' - alpha_r beta)^H (eps' - alpha_r beta))) / beta;
+    fmin over (alpha_r, sqrt(beta)) of beta (sigma_D^2 sigma_HH + sigma_D (sigma_EH + sigma_HE) / beta + sigma_EE)
+    + max(0, beta - 1e-3)^2 from [0, 1] with simplex [[0, 1], [1, 1], [0, 0.9]], xtol = ftol = 0.01,
+    maxiter 100, maxfun 200; alpha_i = the bracket at the optimum; then the kernel-width bias. ``alpha_beta`` skips
+    the search and builds the operators for the given (alpha, beta).
+    """
+    nd = len(shape)
+    eps, xi, zeta, mu = (_matrix_field(_, nd) for _ in (eps, xi, zeta, mu))
+    full = lambda a: np.broadcast_to(a, (a.shape[0], a.shape[1], *shape))
+    if mu.shape[0] == 1:
+        mu_inv = 1.0 / full(mu)
+    else:
+        m = np.moveaxis(full(mu).reshape(3, 3, -1), -1, 0)
+        mu_inv = np.moveaxis(np.linalg.inv(m), 0, -1).reshape(3, 3, *shape)
+    chiEH_beta = -1.0j * _mmul(full(xi), mu_inv)
+    chiHE_beta = 1.0j * _mmul(mu_inv, full(zeta))
+    eps_e = full(eps) - _mmul(full(xi), -1.0j * chiHE_beta)
+    eye = lambda a: np.eye(a.shape[0]).reshape(a.shape[0], a.shape[0], *([1] * nd))
+    sigma_d = float(np.linalg.norm(2.0 * np.pi / (2.0 * np.broadcast_to(np.asarray(pitch, float), (nd,))))) / wavenumber
+    sigma_he = _max_eig_herm(_mmul(_adj(chiHE_beta), chiHE_beta)) ** 0.5
+    sigma_eh = _max_eig_herm(_mmul(_adj(chiEH_beta), chiEH_beta)) ** 0.5
+
+    def sigma_hh(beta):
+        d = mu_inv - beta * eye(mu_inv)
+        return _max_eig_herm(_mmul(_adj(d), d)) ** 0.5 / abs(beta)
+
+    def sigma_ee_m(alpha_r, beta):
+        d = eps_e - alpha_r * beta * eye(eps_e)
+        return _max_eig_herm(_mmul(_adj(d), d)) ** 0.5 / abs(beta)
+
+    def total(eta, beta):
+        return sigma_d ** 2 * sigma_hh(beta) + sigma_d * (sigma_eh + sigma_he) / abs(beta) + sigma_ee_m(eta, beta)
+
+    def target(vec):
+        beta = vec[1] ** 2
+        return total(vec[0], beta) * beta + max(0.0, beta - 1e-3) ** 2
+
+    if alpha_beta is None:
+        vec = scipy.optimize.fmin(target, [0, 1], initial_simplex=[[0, 1], [1, 1], [0, 0.9]], disp=False,
+                                  full_output=False, ftol=0.01, xtol=0.01, maxiter=100, maxfun=200)
+        beta = float(vec[1] ** 2)
+        alpha = complex(vec[0], total(vec[0], beta))
+        alpha = increase_bias(alpha, bound_thickness, wavenumber)
+    else:
+        alpha, beta = complex(alpha_beta[0]), float(alpha_beta[1])
+    f = 1.0j / alpha.imag
+    chiEE = eps_e * (f / beta) - eye(eps_e) * (alpha * f)
+    chiHH = (eye(mu_inv) - mu_inv / beta) * f
+    return alpha, beta, (chiEE, chiEH_beta * (f / beta), chiHE_beta * (f / beta), chiHH), mu_inv, chiHE_beta
+
+
+class MagneticBornOracle:
+    """The iteration of ``solver.py:1064-1098`` with chi E = chi_EE E + chi_HE curl E + curl(chi_EH E + chi_HH curl E)
+    (``solver.py:532-552``) and the rescaling of ``solver.py:514-529`` after a rejected update."""
+
+    def __init__(self, ops, s, alpha, beta, k, E0=None):
+        self.chiEE, self.chiEH, self.chiHE, self.chiHH = (np.asarray(_, dtype=np.complex128) for _ in ops)
+        self.s = np.asarray(s, dtype=np.complex128)
+        self.alpha = complex(alpha)
+        self.beta = float(beta)
+        self.k = [np.asarray(_, dtype=float) for _ in k]
+        self.E = np.zeros(self.s.shape, dtype=np.complex128) if E0 is None else np.array(E0, dtype=np.complex128)
+        self.previous_update_norm = np.inf
+        self.iteration = 0
+
+    def chi_op(self, E):
+        ED = curl(E, self.k)
+        chiE = chi_mul(self.chiEE, E) + chi_mul(self.chiHE, ED)
+        chiH = chi_mul(self.chiEH, E) + chi_mul(self.chiHH, ED)
+        return chiE + curl(chiH, self.k)
+
+    def update(self):
+        d = green_op(self.chi_op(self.E) + self.s, self.k, self.alpha, True)
+        return self.chi_op(d - self.E)
+
+    def update_operators(self, alpha_new):
+        a_old = self.alpha
+        r = a_old.imag / alpha_new.imag
+        self.s = self.s * r
+        self.chiEH = self.chiEH * r
+        self.chiHE = self.chiHE * r
+        self.chiHH = self.chiHH * r
+        chi = self.chiEE / (1.0j / a_old.imag)
+        chi = chi + (a_old - alpha_new) * np.eye(chi.shape[0]).reshape(chi.shape[0], chi.shape[0],
+                                                                         *([1] * (chi.ndim - 2)))
+        self.chiEE = chi * (1.0j / alpha_new.imag)
+        self.alpha = alpha_new
+
+    def step(self):
+        self.iteration += 1
+        d = self.update()
+        current = float(np.linalg.norm(d.ravel()))
+        with np.errstate(divide='ignore', invalid='ignore'):
+            rel = current / self.previous_update_norm
+        if rel < 1:
+            self.E = self.E + d
+            self.previous_update_norm = current
+            return True
+        self.update_operators(self.alpha.real + 1.50j * self.alpha.imag)
+        return False
+
+    @property
+    def relative_residue(self):
+        if self.previous_update_norm > 0:
+            return self.previous_update_norm / float(np.linalg.norm(self.E.ravel()))
+        return 0.0
 
 
 def setup_problem(shape, pitch, wavenumber, eps, current_density=None, source_distribution=None,

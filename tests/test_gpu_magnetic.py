@@ -87,3 +87,52 @@ def test_magnetic_iteration_matches_fused_kernels(anisotropic):
     assert errH <= 1e-5, errH
     with pytest.raises(NotImplementedError):
         mag.refine()
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128], ids=['c64', 'c128'])
+def test_magnetic_trajectory_matches_oracle(dtype):
+    """A medium with varying permeability and bi-isotropy (mu, xi, zeta per voxel): alpha and beta against the
+    oracle's search (solver.py:405-443 restated in NumPy), then the field after each of 10 iterations against
+    oracle/born_oracle.py MagneticBornOracle from the same alpha and beta (relative L2 <= 1e-10 complex128,
+    <= 2e-5 complex64)."""
+    import scipy.constants as const
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    print(f'magnetic trajectory {np.dtype(dtype).name}', flush=True)
+    rng = np.random.default_rng(11)
+    shape = (32, 24)
+    pitch, wavelength = 0.1e-6, 0.5e-6
+    k0 = 2 * np.pi / wavelength
+    grid = mm.Grid(list(shape), pitch)
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2
+    mu = 1.0 + 0.3 * rng.random(shape)
+    xi = 0.05 * (rng.random(shape) + 1j * rng.random(shape))
+    zeta = 0.05 * rng.random(shape)
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, 8, :] = np.exp(-((np.arange(shape[1]) - 12) / 5.0) ** 2)
+    J[2, 8, :] = 0.4j * J[1, 8, :]
+    bound = LinearBound(grid, thickness=0.6e-6)
+    sol = mm.Solution(grid, vacuum_wavelength=wavelength, current_density=J, epsilon=eps, mu=mu, xi=xi, zeta=zeta,
+                      bound=bound, dtype=dtype)
+    assert sol.magnetic
+    eps_b = eps + bound.electric_susceptibility
+    thickness = np.asarray(bound.thickness)
+    a_o, b_o, _, _, _ = ob.magnetic_setup(shape, pitch, k0, eps_b, xi, zeta, mu, thickness.ravel())
+    tol_ab = 1e-9 if dtype == np.complex128 else 1e-4
+    assert abs(sol.alpha - a_o) <= tol_ab * abs(a_o) and abs(sol.beta - b_o) <= tol_ab * abs(b_o), \
+        (sol.alpha, a_o, sol.beta, b_o)
+    alpha, beta = sol.alpha, sol.beta
+    _, _, ops, _, _ = ob.magnetic_setup(shape, pitch, k0, eps_b, xi, zeta, mu, thickness.ravel(),
+                                        alpha_beta=(alpha, beta))
+    S = J.reshape(3, 1, *shape) * (-1j * k0 * const.c * const.mu_0)
+    k = ob.k_vectors(shape, np.array([pitch, pitch]) * k0)
+    o = ob.MagneticBornOracle(ops, S * (1j / alpha.imag / beta), alpha, beta, k)
+    tol = 1e-10 if dtype == np.complex128 else 2e-5
+    for it in range(10):
+        sol.step()
+        o.step()
+        assert sol.alpha == o.alpha
+        Eg = sol.field_array.cpu().numpy()
+        err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
+        assert err <= tol, (it, err)

@@ -12,7 +12,10 @@ Each oracle function is checked against something other than its own formula:
 - mat3_max_eig_hermitian: numpy's general (non-Hermitian) eigen-solver;
 - BornOracle (iteration, accept/reject, residue): the analytic plane-wave solutions of tests/test_solution.py:91-358
   (iteration counts <= 70 / <= 85 to residue 1e-6, decimal=3 field error), the bookkeeping definitions of
-  tests/test_solution.py:59-77, and the divergence guard on an injected under-damped alpha.
+  tests/test_solution.py:59-77, and the divergence guard on an injected under-damped alpha;
+- magnetic_setup / MagneticBornOracle / curl: the reference's negative-index plate (tests/test_solution.py:360-411,
+  golden field norms to 5 decimals), the plain oracle's field for a medium made magnetic by a negligible xi, and the
+  analytic curl of a plane wave.
 """
 import numpy as np
 import numpy.testing as npt
@@ -293,3 +296,88 @@ def test_oracle_divergence_guard():
     # rescaling chi and s in place equals rebuilding them from eps for the new alpha
     npt.assert_allclose(o.chi, ob.chi_base(eps, o.alpha, 1.0), rtol=1e-10, atol=1e-12)
     npt.assert_allclose(o.s, s * (alpha.imag / o.alpha.imag), rtol=1e-10)
+
+
+def _negative_index_plate():
+    """The reference's magnetic test case (tests/test_solution.py:360-398): 128^2 at lambda/4, a 5 um plate of
+    refractive index -1.5 (eps and mu both negative inside), 2 um LinearBound with kappa up to 0.5, a 30-degree
+    beam from a line source near the far boundary."""
+    wavelength = 500e-9
+    k0 = 2 * np.pi / wavelength
+    shape = (128, 128)
+    pitch = wavelength / 4
+    x = [(np.arange(n) - n // 2) * pitch for n in shape]
+    g0, g1 = x[0][:, None], x[1][None, :]
+    ang = np.pi / 6
+    rot = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
+    k_inc = rot * k0 @ np.array([1, 0, 0])
+    pol = (rot @ np.array([0, 1, 1j]) / np.sqrt(2))[:, None, None]
+    J = np.exp(1j * (k_inc[0] * g0 + k_inc[1] * g1))
+    src = shape[0] - int(2e-6 / pitch)
+    J[:src, :] = 0
+    J[src + 1:, :] = 0
+    J = J * np.exp(-0.5 * ((g1 - x[1][shape[0] // 3]) / 2.5e-6) ** 2)
+    J = J[None] * pol
+    n = 1 + (-1.5 - 1) * np.ones((1, 128)) * (np.abs(g0) < 2.5e-6)
+    sign = 1 - 2 * (n < 0)
+    eps = (n ** 2 * sign)[None, None]
+    mu = sign.astype(np.complex128)[None, None]
+    thickness = np.full((2, 2), 2e-6)
+    kappa = ob.linear_bound_extinction(shape, (pitch, pitch), thickness, 0.5)
+    eps = eps + ob.bound_susceptibility(kappa)
+    S = J * (-1j * k0 * const.c * const.mu_0)
+    return shape, pitch, k0, eps, mu, S, src, thickness
+
+
+def test_magnetic_oracle_reproduces_reference_negative_index_norms():
+    """MagneticBornOracle + magnetic_setup against the reference's golden field norms in front of and behind the
+    negative-index plate (tests/test_solution.py:400-411: residue < 1e-3 within 1000 iterations, norms
+    0.0001864 and 0.0002582 to 5 decimals)."""
+    shape, pitch, k0, eps, mu, S, src, thickness = _negative_index_plate()
+    alpha, beta, ops, _, _ = ob.magnetic_setup(shape, pitch, k0, eps, 0.0, 0.0, mu, thickness.ravel())
+    k = ob.k_vectors(shape, np.array([pitch, pitch]) * k0)
+    o = ob.MagneticBornOracle(ops, S.reshape(3, 1, *shape) * (1j / alpha.imag / beta), alpha, beta, k)
+    while True:
+        o.step()
+        if not (o.relative_residue > 1e-3 and o.iteration < 1e4):
+            break
+    assert o.relative_residue < 1e-3 and o.iteration <= 1000, (o.relative_residue, o.iteration)
+    E = o.E[:, 0] / k0 ** 2
+    npt.assert_almost_equal(np.linalg.norm(E[:, -src - 8:-src - 1].ravel()), 0.0001864, decimal=5)
+    npt.assert_almost_equal(np.linalg.norm(E[:, src + 1:src + 8].ravel()), 0.0002582, decimal=5)
+
+
+def test_magnetic_oracle_matches_plain_oracle_for_negligible_coupling():
+    """With xi = 1e-9 at one voxel (magnetic by the reference's definition, physically the same medium), the magnetic
+    iteration converges to the plain iteration's field; also checks curl against the analytic derivative of a
+    plane wave."""
+    rng = np.random.default_rng(3)
+    shape = (24, 20)
+    pitch, k0 = 0.1e-6, 2 * np.pi / 0.5e-6
+    eps = ((1.0 + 0.3 * rng.random(shape)) ** 2)[None, None]
+    thickness = np.full((2, 2), 0.6e-6)
+    eps = eps + ob.bound_susceptibility(ob.linear_bound_extinction(shape, (pitch, pitch), thickness, 0.25))
+    S = np.zeros((3, 1, *shape), dtype=np.complex128)
+    S[1, 0, 6, :] = 1.0
+    S[2, 0, 6, :] = 0.3j
+    k = ob.k_vectors(shape, np.array([pitch, pitch]) * k0)
+    xi = np.zeros((1, 1, *shape), dtype=np.complex128)
+    xi[0, 0, 10, 10] = 1e-9
+    alpha, beta, ops, _, _ = ob.magnetic_setup(shape, pitch, k0, eps, xi, 0.0, 1.0, thickness.ravel())
+    om = ob.MagneticBornOracle(ops, S * (1j / alpha.imag / beta), alpha, beta, k)
+    while True:
+        om.step()
+        assert om.iteration < 20000
+        if om.relative_residue <= 1e-10:
+            break
+    a2, b2 = ob.find_alpha_beta(eps, 1.0, thickness.ravel(), k0)
+    op = ob.solve_to(ob.chi_base(np.broadcast_to(eps, (1, 1, *shape)), a2, b2), S * (1j / a2.imag / b2), a2, b2, k,
+                     True, tol=1e-10)
+    err = np.linalg.norm(om.E - op.E) / np.linalg.norm(op.E)
+    assert err < 1e-7, err
+    # curl of E = (0, exp(i k x), 0) along axis 0 is (0, 0, i k exp(i k x)) in units of k0
+    kk = k[0][3]
+    F = np.zeros((3, 1, *shape), dtype=np.complex128)
+    F[1, 0] = np.exp(1j * kk * (np.arange(shape[0]) * pitch * k0))[:, None]
+    npt.assert_allclose(ob.curl(F, k)[2, 0], 1j * kk * F[1, 0], atol=1e-12)
+    npt.assert_allclose(ob.curl(F, k)[:2], 0, atol=1e-12)
