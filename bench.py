@@ -299,6 +299,64 @@ def smatrix_benchmark(n0=512, nt=128, tol=1e-6, log=lambda m: None):
 
 
 # ---------------------------------------------------------------------------------------------------------------
+# Magnetic medium: the reference's negative-index plate (tests/test_solution.py:360-398), 128^2 vectorial complex64,
+# solved to residue 1e-3 as there; scaled to 512^2 (same physical layout, 4x the pitch count per axis) for timing.
+def magnetic_benchmark(n=512, log=lambda m: None):
+    import scipy.constants as const
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    wavelength = 500e-9
+    k0 = 2 * np.pi / wavelength
+    out = {}
+    for size in (128, n):
+        pitch = wavelength / 4 * 128 / size
+        grid = mm.Grid([size, size], pitch)
+        ang = np.pi / 6
+        rot = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
+        k_inc = rot * k0 @ np.array([1, 0, 0])
+        pol = (rot @ np.array([0, 1, 1j]) / np.sqrt(2))[:, np.newaxis, np.newaxis]
+        J = np.exp(1j * (k_inc[0] * grid[0] + k_inc[1] * grid[1]))
+        src = grid.shape[0] - int(2e-6 / grid.step[0])
+        J[:src, :] = 0
+        J[src + 1:, :] = 0
+        J = J * np.exp(-0.5 * ((grid[1] - grid[1].ravel()[grid.shape[0] // 3]) / 2.5e-6) ** 2)
+        J = J[np.newaxis, ...] * pol
+        nidx = 1 + (-1.5 - 1) * np.ones(grid[1].shape) * (np.abs(grid[0]) < 2.5e-6)
+        bound = LinearBound(grid, thickness=2e-6, max_extinction_coefficient=0.50)
+        sol = mm.Solution(grid, vacuum_wavelength=wavelength, current_density=J, refractive_index=nidx, bound=bound,
+                          dtype=np.complex64)
+        _, t = _timed(lambda: sol.solve(lambda s_: s_.iteration < 1e4 and s_.residue > 1e-3))
+        E = sol.E
+        rec = {'grid': [size, size], 'iterations': int(sol.iteration), 'residue': float(sol.residue),
+               'solve_s': round(t, 3), 'ms_per_iteration': round(1e3 * t / sol.iteration, 3)}
+        if size == 128:
+            rec['front_norm'] = float(np.linalg.norm(E[:, src + 1:src + 8, :].ravel()))
+            rec['back_norm'] = float(np.linalg.norm(E[:, -src - 8:-src - 1, :].ravel()))
+            rec['reference_norms'] = {'front': 0.0002582, 'back': 0.0001864}
+        log(f'magnetic {size}^2: {sol.iteration} iterations, {t:.2f} s')
+        # one complex128 oracle iteration on the host, same alpha and beta
+        sign = 1 - 2 * (nidx < 0)
+        eps = (nidx ** 2 * sign)[None, None] + bound.electric_susceptibility
+        _, _, ops, _, _ = ob.magnetic_setup((size, size), pitch, k0, eps, 0.0, 0.0, sign[None, None].astype(complex),
+                                            alpha_beta=(sol.alpha, sol.beta))
+        S = J.reshape(3, 1, size, size) * (-1j * k0 * const.c * const.mu_0)
+        kv = ob.k_vectors((size, size), np.array([pitch, pitch]) * k0)
+        o = ob.MagneticBornOracle(ops, S * (1j / sol.alpha.imag / sol.beta), sol.alpha, sol.beta, kv)
+        o.step()
+        t0 = time.perf_counter()
+        o.step()
+        rec['cpu_oracle_seconds_per_iteration'] = round(time.perf_counter() - t0, 4)
+        rec['cpu_oracle_extrapolated_s'] = round(rec['cpu_oracle_seconds_per_iteration'] * sol.iteration, 1)
+        out[f'{size}'] = rec
+        del sol
+    out['timed'] = ('Solution.solve to residue 1e-3 (set-up excluded); the magnetic iteration is HIP FFT curls, the '
+                    'HIP 3x3 multiply and element-wise device operations (not fused); CPU: one complex128 oracle '
+                    'iteration (numpy.fft, 1 core) x the GPU iteration count')
+    return out
+
+
+# ---------------------------------------------------------------------------------------------------------------
 # Adjoint gradient: 96^3 isotropic vectorial, complex64; design cube 48^3 (eps uniform in [1, 2.25], seed 5),
 # y-polarised Gaussian current on plane z = 12, loss = sum |E|^2 over a 6^3 box centred 6 pixels behind the design.
 def adjoint_benchmark(n=96, design=48, tol=1e-6, max_iter=10000, n_fd=50, h=1e-3, log=lambda m: None):
@@ -442,6 +500,8 @@ def main():
     ap.add_argument('--no-accuracy', action='store_true', help='skip the complex128 reference solve')
     ap.add_argument('--batched-only', action='store_true', help='only the batched-solve comparison (own JSON line)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient measurement')
+    ap.add_argument('--no-magnetic', action='store_true', help='skip the magnetic-medium measurement')
+    ap.add_argument('--magnetic-only', action='store_true', help='only the magnetic-medium measurement')
     ap.add_argument('--adjoint-only', action='store_true', help='only the adjoint-gradient measurement')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='also run one device-to-device copy of this many MiB (calibrates HBM byte counters)')
@@ -455,6 +515,9 @@ def main():
     log = lambda m: print(f'bench: {m}', file=sys.stderr, flush=True)
     if args.batched_only:
         print(json.dumps({'batched': batched_benchmark(log=log)}), flush=True)
+        return
+    if args.magnetic_only:
+        print(json.dumps({'magnetic': magnetic_benchmark(log=log)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({'adjoint': adjoint_benchmark(log=log)}), flush=True)
@@ -550,6 +613,9 @@ def main():
     if not args.no_adjoint:
         torch.cuda.empty_cache()
         out['adjoint'] = adjoint_benchmark(log=log)
+    if not args.no_magnetic:
+        torch.cuda.empty_cache()
+        out['magnetic'] = magnetic_benchmark(log=log)
     print(json.dumps(out), flush=True)
 
 
