@@ -299,6 +299,70 @@ def smatrix_benchmark(n0=512, nt=128, tol=1e-6, log=lambda m: None):
 
 
 # ---------------------------------------------------------------------------------------------------------------
+# The other single-problem configurations: 256^2 scalar complex128 disc medium to residue 1e-6 and 128^3 isotropic
+# vectorial complex64 (Gaussian-smoothed random index in [1.0, 1.6], seed 1) to residue 1e-5.
+def configs_benchmark(log=lambda m: None):
+    import scipy.ndimage
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    out = {}
+    # 256^2 scalar complex128
+    n, pitch = 256, 125e-9
+    grid = mm.Grid(shape=[n, n], step=pitch)
+    idx = disc_medium(n, 0, pitch=pitch)
+    y = (np.arange(n) - n / 2) * pitch
+    J = np.zeros((1, n, n), dtype=np.complex128)
+    J[0, 40, :] = np.exp(-(y / 3e-6) ** 2)
+    sol = mm.Solution(grid, vacuum_wavelength=500e-9, epsilon=idx ** 2, current_density=J,
+                      bound=LinearBound(grid, thickness=32 * pitch, max_extinction_coefficient=0.2),
+                      dtype=np.complex128)
+    _, t = _timed(lambda: sol.solve(lambda s_: s_.residue > 1e-6 and s_.iteration < 10000))
+    rec = {'grid': [n, n], 'dtype': 'c128', 'residue': float(sol.residue), 'iterations': int(sol.iteration),
+           'solve_s': round(t, 3), 'ms_per_iteration': round(1e3 * t / sol.iteration, 4),
+           'voxel_iter_per_s': round(n * n * sol.iteration / t, 1)}
+    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    sv = sol.source_normalized.cpu().numpy().astype(np.complex128)
+    k = ob.k_vectors((n, n), np.asarray(grid.step) * sol.wavenumber)
+    o = ob.BornOracle(chi, sv, sol.alpha, sol.beta, k, False)
+    o.step()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        o.step()
+    rec['cpu_oracle_seconds_per_iteration'] = round((time.perf_counter() - t0) / 5, 5)
+    out['disc_256x256_scalar_c128'] = rec
+    log(f'configs: 256^2 c128 {sol.iteration} iterations, {t:.3f} s')
+    del sol
+    # 128^3 isotropic vectorial complex64
+    n, pitch = 128, 250e-9
+    grid = mm.Grid(shape=[n, n, n], step=pitch)
+    noise = np.random.default_rng(1).standard_normal((n, n, n))
+    sm = scipy.ndimage.gaussian_filter(noise, 2.0, mode='wrap')
+    idx = 1.0 + 0.6 * (sm - sm.min()) / (sm.max() - sm.min())
+    x = (np.arange(n) - n / 2) * pitch
+    J = np.zeros((3, n, n, n), dtype=np.complex64)
+    J[0, :, :, 20] = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2e-6) ** 2)
+    sol = mm.Solution(grid, vacuum_wavelength=1e-6, epsilon=idx ** 2, current_density=J,
+                      bound=LinearBound(grid, thickness=16 * pitch), dtype=np.complex64)
+    _, t = _timed(lambda: sol.solve(lambda s_: s_.residue > 1e-5 and s_.iteration < 10000))
+    rec = {'grid': [n, n, n], 'dtype': 'c64', 'residue': float(sol.residue), 'iterations': int(sol.iteration),
+           'solve_s': round(t, 3), 'ms_per_iteration': round(1e3 * t / sol.iteration, 4),
+           'voxel_iter_per_s': round(n ** 3 * sol.iteration / t, 1)}
+    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    sv = sol.source_normalized.cpu().numpy().astype(np.complex128)
+    k = ob.k_vectors((n, n, n), np.asarray(grid.step) * sol.wavenumber)
+    o = ob.BornOracle(chi, sv, sol.alpha, sol.beta, k, True)
+    t0 = time.perf_counter()
+    o.step()
+    rec['cpu_oracle_seconds_per_iteration'] = round(time.perf_counter() - t0, 3)
+    out['smooth_128cubed_iso_c64'] = rec
+    log(f'configs: 128^3 c64 {sol.iteration} iterations, {t:.3f} s')
+    del sol
+    out['timed'] = 'Solution.solve to the residue (set-up excluded); CPU: complex128 oracle iteration, numpy.fft, 1 core'
+    return out
+
+
+# ---------------------------------------------------------------------------------------------------------------
 # Magnetic medium: the reference's negative-index plate (tests/test_solution.py:360-398), 128^2 vectorial complex64,
 # solved to residue 1e-3 as there; scaled to 512^2 (same physical layout, 4x the pitch count per axis) for timing.
 def magnetic_benchmark(n=512, log=lambda m: None):
@@ -502,6 +566,8 @@ def main():
     ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient measurement')
     ap.add_argument('--no-magnetic', action='store_true', help='skip the magnetic-medium measurement')
     ap.add_argument('--magnetic-only', action='store_true', help='only the magnetic-medium measurement')
+    ap.add_argument('--no-configs', action='store_true', help='skip the 256^2 and 128^3 configurations')
+    ap.add_argument('--configs-only', action='store_true', help='only the 256^2 and 128^3 configurations')
     ap.add_argument('--adjoint-only', action='store_true', help='only the adjoint-gradient measurement')
     ap.add_argument('--copy-probe-mb', type=int, default=0,
                     help='also run one device-to-device copy of this many MiB (calibrates HBM byte counters)')
@@ -518,6 +584,9 @@ def main():
         return
     if args.magnetic_only:
         print(json.dumps({'magnetic': magnetic_benchmark(log=log)}), flush=True)
+        return
+    if args.configs_only:
+        print(json.dumps({'configs': configs_benchmark(log=log)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({'adjoint': adjoint_benchmark(log=log)}), flush=True)
@@ -616,6 +685,9 @@ def main():
     if not args.no_magnetic:
         torch.cuda.empty_cache()
         out['magnetic'] = magnetic_benchmark(log=log)
+    if not args.no_configs:
+        torch.cuda.empty_cache()
+        out['configs'] = configs_benchmark(log=log)
     print(json.dumps(out), flush=True)
 
 
