@@ -287,3 +287,44 @@ def test_deposition_matrix(vectorial, idx):
     row = np.asarray(d1 @ np.eye(s.shape[1])).ravel()
     expected = np.array([centre(s.srcvec2freespace(np.eye(s.shape[1])[j])) for j in range(s.shape[1])])
     npt.assert_array_almost_equal(row, expected, decimal=2)
+
+
+def test_matrix_api_edges():
+    """Caching on/off, a precomputed array, element access, out= arguments and mismatched shapes."""
+    from macromax_amd import matrix
+    print('matrix api edges', flush=True)
+    m, grid, _ = _matrix(False, 0, n=_thin_film(_cases()[0][0]))
+    arr = np.asarray(m)
+    # element access computes only the selected columns; rows come from the full matrix
+    npt.assert_array_almost_equal(m[1, 2], arr[1, 2], decimal=12)
+    npt.assert_array_almost_equal(m[:, 3], arr[:, 3], decimal=12)
+    npt.assert_array_almost_equal(m[2], arr[2], decimal=12)
+    # no caching: every product solves (same numbers to the solver tolerance)
+    m2, _, _ = _matrix(False, 0, n=_thin_film(grid), caching=False)
+    assert not m2.caching
+    v = np.zeros(m2.shape[1], dtype=np.complex128)
+    v[1] = 2.0
+    npt.assert_array_almost_equal(m2 @ v, 2.0 * arr[:, 1], decimal=8)
+    # a precomputed array pre-fills the cache: products need no solve
+    m3, _, _ = _matrix(False, 0, n=_thin_film(grid), array=arr)
+    npt.assert_array_equal(m3 @ v, 2.0 * arr[:, 1])
+    with pytest.raises(ValueError):
+        _matrix(False, 0, array=np.zeros((3, 3)))
+    with pytest.raises(ValueError):  # scipy's LinearOperator checks the shape first
+        m @ np.zeros(m.shape[1] + 1)
+    with pytest.raises(KeyError):
+        m[0, 0] = 1.0
+    # out= arguments
+    src = np.empty((1, *grid.shape), dtype=m.dtype)
+    res = m.srcvec2source(v, out=src)
+    assert res is src and np.any(src != 0)
+    fld = np.empty((1, *grid.shape), dtype=m.dtype)
+    res = m.srcvec2detfield(v, out=fld)
+    assert res is fld
+    # v enters from the front: its incident wave is removed at the front detection plane only
+    back_det, front_det = m.detector_planes
+    out = fld.copy()
+    out[:, front_det] -= m.srcvec2freespace(v)[:, front_det]
+    npt.assert_array_almost_equal(m.detfield2detvec(out), m @ v, decimal=6)
+    # the transfer matrix converts back
+    npt.assert_array_almost_equal(matrix.convert(m.transfer()), arr, decimal=8)
