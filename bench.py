@@ -78,7 +78,8 @@ def setup_solution(n, device, dtype=np.complex64):
 
 
 def cpu_baseline(sol, n):
-    """One complex128 NumPy oracle iteration of the same problem on this host (numpy.fft is single-threaded)."""
+    """Complex128 NumPy oracle iterations of the same problem on this host (numpy.fft is single-threaded): a bounded
+    sample of at least 10 s of CPU work (whole iterations, at least 2, at most 8), reported per iteration."""
     from oracle import born_oracle as ob
     chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
     s = sol.source_normalized.cpu().numpy().astype(np.complex128)
@@ -86,11 +87,15 @@ def cpu_baseline(sol, n):
     k = ob.k_vectors((n, n, n), np.asarray(sol.grid.step) * sol.wavenumber)
     o = ob.BornOracle(chi, s, sol.alpha, sol.beta, k, True, E0=E)
     t0 = time.perf_counter()
-    o.step()
-    dt = time.perf_counter() - t0
+    iters = 0
+    while iters < 8 and (iters < 2 or time.perf_counter() - t0 < 10.0):
+        o.step()
+        iters += 1
+    total = time.perf_counter() - t0
+    dt = total / iters
     return {'value': n ** 3 / dt, 'unit': 'voxel-iter/s', 'cores': 1, 'kind': 'port',
-            'sample': f'1 complex128 iteration of the same {n}^3 anisotropic problem (oracle/born_oracle.py, '
-                      f'numpy.fft), {dt:.1f} s', 'seconds_per_iteration': dt,
+            'sample': f'{iters} complex128 iterations of the same {n}^3 anisotropic problem (oracle/born_oracle.py, '
+                      f'numpy.fft), {total:.1f} s', 'seconds_per_iteration': dt,
             'os_cpu_count': os.cpu_count(), 'affinity_cores': len(os.sched_getaffinity(0))}
 
 
