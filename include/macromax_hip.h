@@ -61,6 +61,9 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *                        1: its transpose, for adjoint solves by reciprocity: the Nyquist entry of every
  *                        even-length axis is +pi/d instead of -pi/d (only the dyadic Green's cross terms
  *                        k_i k_j / k^2 on those planes change). Synchronises the device when it changes.
+ *   MMX_PARAM_SCRATCH_GEN read-only (mmx_plan_get_param): how many times the plan's device scratch buffer (FP64
+ *                        partial sums) has been allocated. Launches recorded in a graph hold that buffer's address,
+ *                        so a caller replaying a graph re-records it when this number has changed since capture.
  */
 #define MMX_PARAM_G_AXIS 1
 #define MMX_PARAM_P_ROWS 2
@@ -71,6 +74,7 @@ int mmx_plan_destroy(mmx_plan_t plan);
 #define MMX_PARAM_BIG_ELEMS 7
 #define MMX_PARAM_REG_FFT 8
 #define MMX_PARAM_TRANSPOSED 9
+#define MMX_PARAM_SCRATCH_GEN 10
 int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
 /*
  * Support of the sources s of the Born calls that follow on this plan: s is zero outside the box

@@ -287,6 +287,7 @@ class BackEndHip(BackEnd):
         self.__k = None
         self.__k2 = None
         self.__eye = None
+        self.__ik3 = None
 
     # ------------------------------------------------------------------ properties
     @property
@@ -536,8 +537,19 @@ class BackEndHip(BackEnd):
         return F - self.longitudinal_projection_ft(F)
 
     def curl_ft(self, field_array_ft) -> torch.Tensor:
-        """i k x F (``backend/__init__.py:483-509``); readout only."""
+        """i k x F (``backend/__init__.py:483-509``).
+
+        A vector field [3, 1, *grid] takes one cross-product kernel with the cached i k [3, 1, *grid] (components
+        along absent axes are 0); other shapes take the component-wise form."""
         F = self.astype(field_array_ft)
+        full = tuple(int(_) for _ in self.grid.shape)
+        if F.ndim == 2 + len(full) and tuple(F.shape[:2]) == (3, 1) and tuple(F.shape[2:]) == full:
+            if self.__ik3 is None:
+                ik = torch.zeros((3, 1, *full), dtype=self.hardware_dtype, device=self.__device)
+                for a in range(self.grid.ndim):
+                    ik[a, 0] = 1j * self.k[a].to(self.hardware_dtype)
+                self.__ik3 = ik
+            return torch.linalg.cross(self.__ik3, F, dim=0)
         k = [1j * _.to(self.hardware_dtype) for _ in self.k] + [0] * (3 - self.grid.ndim)
         comps = [F[c, 0] if c < F.shape[0] else 0 for c in range(3)]
         out = self.allocate_array(shape=(3, *F.shape[1:]))

@@ -63,6 +63,7 @@ struct mmx_plan_s {
     int k1ofs, k2ofs;
     void* scratch;   // FP64 partial sums
     size_t scratch_bytes;
+    int64_t scratch_gen;  // number of (re)allocations of scratch: work captured in a graph holds its address
     int device;
     size_t lds_max;  // device limit of shared memory per block
     // tuning (0 = automatic)
@@ -144,6 +145,7 @@ int ensure_scratch(mmx_plan_s* p, size_t bytes) {
     p->scratch_bytes = 0;
     MMX_HIP_CHECK(hipMalloc(&p->scratch, bytes));
     p->scratch_bytes = bytes;
+    ++p->scratch_gen;
     return MMX_OK;
 }
 
@@ -917,6 +919,7 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
             }
             break;
         }
+        case MMX_PARAM_SCRATCH_GEN: return fail(MMX_EINVAL, "MMX_PARAM_SCRATCH_GEN is read-only");
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;
@@ -935,6 +938,7 @@ int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
         case MMX_PARAM_BIG_ELEMS: *value = p->big_elems; break;
         case MMX_PARAM_REG_FFT: *value = p->reg_fft; break;
         case MMX_PARAM_TRANSPOSED: *value = p->transposed; break;
+        case MMX_PARAM_SCRATCH_GEN: *value = p->scratch_gen; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;

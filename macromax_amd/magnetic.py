@@ -107,11 +107,14 @@ class MagneticOperator:
         self.mu_inv = prep(mu_inv)
         self.chiHE_beta = prep(chiHE_beta)
         self.s = None
+        self._g = None  # (alpha, g) of the Green's operator
+        self.version = 0  # incremented whenever s, alpha or a susceptibility tensor is replaced
 
     # ------------------------------------------------------------------ operators
     def set_source(self, S: torch.Tensor):
         """s = S (i/alpha_i)/beta for the current alpha, [3, 1, *grid]."""
         self.s = (S.to(torch.complex128) * (1.0j / self.alpha.imag / self.beta)).to(self.BE.hardware_dtype).contiguous()
+        self.version += 1
 
     def source_distribution(self) -> torch.Tensor:
         return self.s * (self.beta * self.alpha.imag / 1.0j)
@@ -135,7 +138,9 @@ class MagneticOperator:
         BE = self.BE
         a = self.alpha
         F = BE.ft(X)
-        g = ((-1.0j * a.imag) / (BE.k2.to(torch.complex128) - a)).to(BE.hardware_dtype)
+        if self._g is None or self._g[0] != a:  # g depends only on alpha: kept until alpha changes
+            self._g = (a, ((-1.0j * a.imag) / (BE.k2.to(torch.complex128) - a)).to(BE.hardware_dtype))
+        g = self._g[1]
         PL = BE.longitudinal_projection_ft(F)
         F = g * (F - PL) + PL * complex(1.0j * a.imag / a)
         return BE.ift(F)
@@ -160,6 +165,7 @@ class MagneticOperator:
             chiEE = chiEE + eye3 * (a_old - alpha_new)
         self.chiEE = (chiEE * (1.0j / alpha_new.imag)).to(self.BE.hardware_dtype).contiguous()
         self.alpha = alpha_new
+        self.version += 1
 
     def H(self, E: torch.Tensor, c_mu0: float) -> torch.Tensor:
         """i/(mu_0 c) (-mu^-1 curl E + chi_HE,beta E) for the SI field E [3, 1, *grid] (``solver.py:862-884``)."""

@@ -50,6 +50,10 @@ def solve(grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_w
 class Solution:
     """State of a Born-series solution; iterate it with ``solve()`` or ``for s in solution``."""
 
+    #: magnetic media: record each iteration's kernels (about 50 launches) in a HIP graph and replay it (False: launch
+    #: them one by one; the results are identical)
+    magnetic_graphs = True
+
     def __init__(self, grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_wavelength=None,
                  current_density=None, source_distribution=None, epsilon=None, xi=0.0, zeta=0.0, mu=1.0,
                  refractive_index=None, bound: Bound | None = None, initial_field=0.0, dtype=None):
@@ -116,6 +120,10 @@ class Solution:
         self.__norms_host = [torch.zeros(2, dtype=torch.float64, pin_memory=True) for _ in range(2)]
         self.__field_fp64 = None
         self.__BE64 = None
+        self.__mag_graphs = {}  # (input buffer, output buffer) -> graph of one magnetic iteration
+        self.__mag_graph_state = None  # (operator version, plan scratch generation) the graphs were recorded for
+        self.__mag_pool = None
+        self.__mag_norms = None
         self.E = initial_field
 
         def media_field(f):  # mu, xi, zeta as complex128 [1|3, 1|3, *grid-broadcastable] device tensors
@@ -643,23 +651,69 @@ class Solution:
         self.__update_operators(alpha_new)  # also voids the iteration launched ahead and the work buffer
         return False
 
+    def __magnetic_compute(self, cur: int, nxt: int, norms: torch.Tensor):
+        """d = chi (G (chi E + s) - E) for E in field buffer cur; buffer nxt = E + d; norms = (|d|^2, |E + d|^2) in
+        FP64 (``solver.py:1070-1082``)."""
+        E = self.__fields[cur]
+        d = self.__mag.update(E).contiguous()
+        torch.add(E, d, out=self.__fields[nxt])
+        plan = self.__BE.plan
+        plan.norm2(d, norms[0:1])
+        plan.norm2(self.__fields[nxt], norms[1:2])
+
+    def __magnetic_launch(self, cur: int, nxt: int) -> torch.Tensor:
+        """Enqueue one magnetic iteration; returns the device tensor that will hold its two squared norms.
+
+        With ``magnetic_graphs``, the iteration's launches are recorded once per (input, output) buffer pair in a HIP
+        graph (torch.cuda.CUDAGraph, one memory pool for all of them) and replayed. The graphs hold the operator's
+        tensors, alpha and the plan's scratch address, so they are recorded again when the operator changes (new
+        alpha or source) or the plan's scratch buffer has been reallocated. The first iteration after such a change
+        runs eagerly, which also creates every lazily built constant outside the graph's memory pool."""
+        if self.__mag_norms is None:
+            self.__mag_norms = torch.zeros(2, dtype=torch.float64, device=self.__BE.device)
+        norms = self.__mag_norms
+        if not self.magnetic_graphs:
+            self.__magnetic_compute(cur, nxt, norms)
+            return norms
+        plan = self.__BE.plan
+        state = (self.__mag.version, plan.get_param('scratch_gen'))
+        if state != self.__mag_graph_state:
+            self.__mag_graphs = {}
+            self.__mag_pool = None
+            self.__mag_graph_state = state
+            self.__magnetic_compute(cur, nxt, norms)  # eager
+            return norms
+        key = (cur, nxt, self.__fields[cur].data_ptr(), self.__fields[nxt].data_ptr())
+        g = self.__mag_graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, pool=self.__mag_pool):
+                self.__magnetic_compute(cur, nxt, norms)
+            if self.__mag_pool is None:
+                self.__mag_pool = g.pool()
+            if plan.get_param('scratch_gen') != state[1]:  # recording grew the scratch: the graph is stale
+                self.__mag_graph_state = None
+                self.__magnetic_compute(cur, nxt, norms)
+                return norms
+            self.__mag_graphs[key] = g
+        g.replay()
+        return norms
+
     def __magnetic_step(self) -> bool:
         """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``)."""
         self.__iteration += 1
         self.__relative_residue = None
-        BE = self.__BE
-        E = self.__fields[self.__cur]
-        d = self.__mag.update(E)
-        current = BE.norm(d)
+        cur = self.__cur
+        nxt = (cur + 1) % 3
+        d2, e2 = self.__magnetic_launch(cur, nxt).tolist()
+        current = math.sqrt(d2)
         with np.errstate(divide='ignore', invalid='ignore'):
             rel = np.float64(current) / np.float64(self.__previous_update_norm)
         if rel < 1:
-            nxt = (self.__cur + 1) % 3
-            torch.add(E, d, out=self.__fields[nxt])
             self.__cur = nxt
             self.__field_fp64 = None
             self.__previous_update_norm = current
-            self.__field_norm = BE.norm(self.__fields[nxt])
+            self.__field_norm = math.sqrt(e2)
             return True
         log.warning(f'The field update is scaled by {rel:0.3f} >= 1 in iteration {self.__iteration}.')
         alpha_new = complex(self.__alpha.real, 1.50 * self.__alpha.imag)

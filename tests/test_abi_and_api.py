@@ -37,6 +37,23 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert L.mmx_version() >= 100
 
 
+def test_header_constants_match_the_binding():
+    """Every MMX_PARAM_*, MMX_STATE_* and MMX_TRACK_* value of the header is the one the ctypes binding uses."""
+    from macromax_amd import _native
+    text = open(os.path.join(ROOT, 'include', 'macromax_hip.h')).read()
+    defs = {m.group(1): int(m.group(2)) for m in re.finditer(r'#define\s+(MMX_\w+)\s+(-?\d+)u?\b', text)}
+
+    def group(prefix):
+        return {k[len(prefix):].lower(): v for k, v in defs.items() if k.startswith(prefix)}
+    assert group('MMX_PARAM_') == _native.PARAMS
+    track = group('MMX_TRACK_')
+    assert track.pop('len') == _native.TRACK_LEN
+    assert track == _native.TRACK
+    assert (defs['MMX_STATE_PARITY'], defs['MMX_STATE_DONE'], defs['MMX_STATE_STALE']) == \
+        (_native.STATE_PARITY, _native.STATE_DONE, _native.STATE_STALE)
+    assert defs['MMX_C64'] == _native.MMX_C64 and defs['MMX_C128'] == _native.MMX_C128
+
+
 def test_backend_load_without_gpu_fails_like_the_reference():
     import torch
     from macromax_amd import backend

@@ -136,3 +136,44 @@ def test_magnetic_trajectory_matches_oracle(dtype):
         Eg = sol.field_array.cpu().numpy()
         err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
         assert err <= tol, (it, err)
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128], ids=['c64', 'c128'])
+def test_magnetic_graph_replays_match_eager_launches(dtype):
+    """The magnetic iteration replayed from HIP graphs (Solution.magnetic_graphs) gives the same fields, norms and
+    alpha as the same iteration launched kernel by kernel, over 14 iterations that include eager work and host
+    reads between replays (derived fields, norms) and a rejected update (the field scaled by 50 in place), after
+    which alpha changes and the graphs are recorded again."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print(f'magnetic graphs vs eager {np.dtype(dtype).name}', flush=True)
+    rng = np.random.default_rng(12)
+    shape = (32, 24)
+    grid = mm.Grid(list(shape), 0.1e-6)
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2
+    mu = 1.0 + 0.3 * rng.random(shape)
+    xi = 0.05 * (rng.random(shape) + 1j * rng.random(shape))
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, 8, :] = np.exp(-((np.arange(shape[1]) - 12) / 5.0) ** 2)
+    bound = LinearBound(grid, thickness=0.6e-6)
+
+    def make(graphs):
+        s = mm.Solution(grid, vacuum_wavelength=0.5e-6, current_density=J, epsilon=eps, mu=mu, xi=xi,
+                        bound=bound, dtype=dtype)
+        s.magnetic_graphs = graphs
+        return s
+    eager, graphed = make(False), make(True)
+    assert graphed.magnetic and graphed.alpha == eager.alpha
+    rejected = 0
+    for it in range(14):
+        if it == 8:
+            for s in (eager, graphed):
+                s.field_array.mul_(50.0)  # in place: the next update is larger than the last one
+        acc_e, acc_g = eager.step(), graphed.step()
+        assert acc_e == acc_g, it
+        rejected += not acc_g
+        npt.assert_array_equal(graphed.field_array.cpu().numpy(), eager.field_array.cpu().numpy(), err_msg=str(it))
+        assert graphed.residue == eager.residue and graphed.alpha == eager.alpha, it
+        if it % 3 == 1:
+            npt.assert_array_equal(graphed.H, eager.H)
+    assert rejected >= 1
