@@ -357,10 +357,14 @@ int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, i
     // single-component strided passes; 8 columns of 3 components (16 of 1) for the Green pass
     int want = p_request > 0 ? p_request : (rows ? (NC == 3 ? 8 : 16) : (fused ? (NC == 3 ? 8 : 16) : 32));
     if (want > pmax) want = pmax;
+    // Small grids: more, smaller tiles fill more CUs, but single-component passes measured fastest with at least 4
+    // rows per tile and, for strided tiles, at least 16 columns (128-byte row segments) - e.g. a 512^2 3-component
+    // FFT 28 us with 1-row / 2-column tiles, 15 us with 8 / 16 (fused passes keep the smaller minimum).
+    const int rows_min = fused ? 1 : 4, cols_min = fused ? 2 : 16;
     if (rows) {
         const int64_t total_rows = t.nouter * mult;
         if (p_request <= 0) {
-            while (want > 1 && total_rows / want < 1024) want >>= 1;  // keep >= ~1024 tiles when possible
+            while (want > rows_min && total_rows / want < 1024) want >>= 1;  // keep >= ~1024 tiles when possible
             while (want > 1 && want > t.nouter) want >>= 1;
         }
         t.P = pow2floor(want);
@@ -370,7 +374,7 @@ int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, i
         int P = pow2floor(want);
         while (P > 1 && P > ax.inner) P >>= 1;
         if (p_request <= 0)
-            while (P > 2 && t.nouter * mult * ((ax.inner + P - 1) / P) < 512) P >>= 1;
+            while (P > cols_min && t.nouter * mult * ((ax.inner + P - 1) / P) < 512) P >>= 1;
         t.P = P;
         t.ntile_inner = (ax.inner + P - 1) / P;
         t.tiles_per_problem = t.nouter * t.ntile_inner;
