@@ -605,6 +605,27 @@ class BackEndHip(BackEnd):
             return arr[:, 0].clone()
         return hermitian3_eigvals(arr).to(self.hardware_dtype)
 
+    def calc_roots_of_low_order_polynomial(self, C) -> torch.Tensor:
+        """Roots of polynomials of degree <= 3 in parallel (``backend/__init__.py:725-850``).
+
+        C holds the coefficients from low to high order along axis 0, one polynomial per position along the other
+        axes: sum_k C[k] x^k == 0. The result holds the roots along axis 0 (one entry fewer than C), sorted by real
+        then imaginary part. When the leading coefficients are below 2 eps of the hardware type the polynomial has
+        fewer roots; the missing ones are nan and sort last. Computed on the device in complex128 (see
+        ``polynomial_roots``) and returned in the hardware type.
+        """
+        Ct = torch.as_tensor(np.asarray(C) if not isinstance(C, torch.Tensor) else C, device=self.__device)
+        return polynomial_roots(Ct.to(torch.complex128), 2.0 * self.eps).to(self.hardware_dtype)
+
+    @staticmethod
+    def evaluate_polynomial(C, X):
+        """sum_k C[k] X^k per polynomial (Horner's scheme; ``backend/__init__.py:853-866``), for NumPy arrays or
+        tensors."""
+        result = 0
+        for k in range(C.shape[0] - 1, -1, -1):
+            result = result * X + C[k]
+        return result
+
     @staticmethod
     def clear_cache():
         pass
@@ -634,6 +655,86 @@ class BackEndHip(BackEnd):
         return f'BackEndHip(vectorial={self.vectorial}, {self.grid}, dtype={self.numpy_dtype}, device={self.__device})'
 
     __repr__ = __str__
+
+
+def polynomial_roots(C: torch.Tensor, tol: float) -> torch.Tensor:
+    """Roots of the polynomials sum_k C[k] x^k (degree <= 3, complex128 [n, ...]) -> [n - 1, ...] complex128.
+
+    The degree of each polynomial is the highest k with |C[k]| >= tol; its missing roots are nan. Monic linear:
+    x = -c0. Monic quadratic: q = -(b + sigma sqrt(b^2 - 4 c0)) / 2 with the sign sigma that avoids cancellation,
+    roots q and c0 / q (Numerical Recipes' form). Monic cubic: x = t - a / 3 turns it into t^3 + p t + q = 0;
+    Cardano's u = cbrt(-q/2 +- sqrt(q^2/4 + p^3/27)) with the sign of larger magnitude and v = -p / (3 u) give
+    t = w u + v / w for the three cube roots of unity w (t = 0 three times when u = 0). Every root then takes two
+    Newton steps on the original polynomial, each kept only if it lowers |P(x)|. The roots of each polynomial are
+    sorted lexicographically (real, then imaginary part; nan last).
+    """
+    n = C.shape[0]
+    rest = tuple(C.shape[1:])
+    if n < 2:
+        return torch.empty((0, *rest), dtype=torch.complex128, device=C.device)
+    if n > 4:
+        raise ValueError('Orders above 3 not implemented!')
+    d = n - 1
+    P = C.reshape(n, -1)
+    M = P.shape[1]
+    deg = torch.zeros(M, dtype=torch.int64, device=C.device)
+    for k in range(1, n):
+        deg = torch.where(P[k].abs() >= tol, torch.full_like(deg, k), deg)
+    nan = complex(float('nan'), float('nan'))
+    roots = torch.full((d, M), nan, dtype=torch.complex128, device=C.device)
+
+    def horner(c, x):  # c: [e + 1, K] low to high, x: [r, K]
+        val = torch.zeros_like(x)
+        der = torch.zeros_like(x)
+        for k in range(c.shape[0] - 1, -1, -1):
+            der = der * x + val
+            val = val * x + c[k]
+        return val, der
+
+    for e in range(1, d + 1):
+        sel = deg == e
+        if not bool(sel.any()):
+            continue
+        c = P[:e + 1, sel] / P[e, sel]  # monic
+        if e == 1:
+            r = -c[0:1]
+        elif e == 2:
+            b, c0 = c[1], c[0]
+            sq = torch.sqrt(b * b - 4.0 * c0)
+            sigma = torch.where((b.conj() * sq).real >= 0, 1.0, -1.0).to(torch.complex128)
+            q = -0.5 * (b + sigma * sq)
+            q_zero = q == 0
+            r2 = torch.where(q_zero, torch.zeros_like(q), c0 / torch.where(q_zero, torch.ones_like(q), q))
+            r = torch.stack([q, r2])
+        else:
+            a, b, c0 = c[2], c[1], c[0]
+            p = b - a * a / 3.0
+            q = 2.0 * a ** 3 / 27.0 - a * b / 3.0 + c0
+            sd = torch.sqrt(q * q / 4.0 + p ** 3 / 27.0)
+            w1, w2 = -q / 2.0 + sd, -q / 2.0 - sd
+            w = torch.where(w1.abs() >= w2.abs(), w1, w2)
+            u = torch.pow(w, 1.0 / 3.0)
+            u_zero = u == 0
+            v = torch.where(u_zero, torch.zeros_like(u), -p / (3.0 * torch.where(u_zero, torch.ones_like(u), u)))
+            unity = torch.tensor([1.0, complex(-0.5, math.sqrt(3) / 2), complex(-0.5, -math.sqrt(3) / 2)],
+                                 dtype=torch.complex128, device=C.device)[:, None]
+            r = unity * u[None] + unity.conj() * v[None] - a[None] / 3.0
+        for _ in range(2):  # Newton polish, kept where it lowers |P|
+            val, der = horner(c, r)
+            ok = der != 0
+            step = torch.where(ok, val / torch.where(ok, der, torch.ones_like(der)), torch.zeros_like(val))
+            r_new = r - step
+            better = horner(c, r_new)[0].abs() < val.abs()
+            r = torch.where(better, r_new, r)
+        block = torch.full((d, r.shape[1]), nan, dtype=torch.complex128, device=C.device)
+        block[:e] = r
+        roots[:, sel] = block
+    # lexicographic sort along axis 0: stable sort by imaginary part, then by real part (nan sorts last)
+    order = torch.sort(roots.imag, dim=0, stable=True).indices
+    roots = torch.gather(roots, 0, order)
+    order = torch.sort(roots.real, dim=0, stable=True).indices
+    roots = torch.gather(roots, 0, order)
+    return roots.reshape(d, *rest)
 
 
 def hermitian3_eigvals(m: torch.Tensor) -> torch.Tensor:

@@ -204,3 +204,25 @@ def test_load_selects_hip(device):
     assert isinstance(be, BackEndHip) and not be.vectorial
     with pytest.raises(ValueError):
         backend.load(1, Grid([11, 8], 1.0), np.complex64)  # prime factor 11: explicit error, no fallback
+
+
+def test_calc_roots_of_low_order_polynomial(BE):
+    """``python/tests/test_backend_numpy.py:12-161``: the reference's known answers, computed on the device."""
+    import torch
+    from polynomial_cases import CASES, MULTI
+    for C, expected, decimal in CASES:
+        roots = BE.calc_roots_of_low_order_polynomial(np.asarray(C))
+        assert isinstance(roots, torch.Tensor) and roots.device.type == 'cuda' and roots.dtype == torch.complex64
+        npt.assert_array_almost_equal(BE.asnumpy(roots), np.asarray(expected, dtype=np.complex64), decimal=decimal,
+                                      err_msg=str(C))
+    for C, expected in MULTI:
+        npt.assert_array_almost_equal(BE.asnumpy(BE.calc_roots_of_low_order_polynomial(np.asarray(C))),
+                                      np.asarray(expected, dtype=np.complex64), err_msg=str(C))
+    assert BE.asnumpy(BE.calc_roots_of_low_order_polynomial(np.array([1]))).shape == (0,)
+    C = np.array([-1, 0, 0, 1])  # x**3 == 1
+    roots = np.round(BE.asnumpy(BE.calc_roots_of_low_order_polynomial(C)), decimals=6)
+    npt.assert_array_almost_equal(np.sort(roots), [np.exp(-2j * np.pi / 3), np.exp(2j * np.pi / 3), 1])
+    C = np.array([1, 2, 3, -4])
+    roots = BE.calc_roots_of_low_order_polynomial(BE.astype(C))
+    for r in range(3):
+        npt.assert_almost_equal(BE.asnumpy(BE.evaluate_polynomial(BE.astype(C), roots[r])), 0.0, decimal=5)
