@@ -226,3 +226,44 @@ def test_calc_roots_of_low_order_polynomial(BE):
     roots = BE.calc_roots_of_low_order_polynomial(BE.astype(C))
     for r in range(3):
         npt.assert_almost_equal(BE.asnumpy(BE.evaluate_polynomial(BE.astype(C), roots[r])), 0.0, decimal=5)
+
+
+def test_cross_and_outer(BE):
+    """``python/tests/test_backend.py`` test_cross and test_outer known answers."""
+    A, B = BE.astype(np.array([1, 2, 3])), BE.astype(np.array([3, 4, 5]))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.cross(A, B)), np.array([-2, 4, -2]))
+    A, B = BE.astype(np.array([1, 2, 3]).reshape(3, 1, 1)), BE.astype(np.array([3, 4, 5]).reshape(3, 1, 1))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.cross(A, B)), np.array([-2, 4, -2]).reshape(3, 1, 1))
+    A, B = BE.astype(np.array([1, 2]).reshape(2, 1)), BE.astype(np.array([3, 4, 5]).reshape(3, 1))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.outer(A, B)), np.array([[3, 4, 5], [6, 8, 10]]))
+    A, B = BE.astype(np.array([1, 2]).reshape(2, 1, 1)), BE.astype(np.array([3, 4, 5]).reshape(3, 1, 1))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.outer(A, B)), np.array([[3, 4, 5], [6, 8, 10]]).reshape(2, 3, 1))
+    # outer is A B^H: the second factor is conjugated
+    A, B = BE.astype(np.array([1j, 2]).reshape(2, 1)), BE.astype(np.array([1j, 1]).reshape(2, 1))
+    npt.assert_array_almost_equal(BE.asnumpy(BE.outer(A, B)), np.array([[1, 1j], [-2j, 2]]))
+
+
+def test_div_ft_and_div(BE):
+    """``python/tests/test_backend.py`` test_div_ft and test_div (constant fields have no divergence), plus a plane
+    wave: div(a exp(i k.x)) = i (k.a) exp(i k.x) for a grid k-vector k."""
+    shape = BE.grid.shape
+    V = np.zeros([3, 1, *shape])
+    V[:, 0, 0, 0, 0] = [1.0, 2.0, 3.0]
+    npt.assert_array_almost_equal(BE.asnumpy(BE.div_ft(BE.astype(V))), np.zeros([1, 1, *shape]))
+    M = np.zeros([3, 3, *shape])
+    M[:, :, 0, 0, 0] = np.arange(9).reshape(3, 3)
+    npt.assert_array_almost_equal(BE.asnumpy(BE.div_ft(BE.astype(M))), np.zeros([3, 1, *shape]))
+    F = np.zeros([3, 1, *shape])
+    F[0], F[1], F[2] = 1.0, 2.0, 3.0
+    npt.assert_almost_equal(BE.asnumpy(BE.div(BE.astype(F))), np.zeros([1, 1, *shape]), decimal=6)
+    M = np.tile(np.arange(9).reshape([3, 3, 1, 1, 1]), [1, 1, *shape])
+    npt.assert_almost_equal(BE.asnumpy(BE.div(BE.astype(M))), np.zeros([3, 1, *shape]), decimal=6)
+    # plane wave along a grid frequency (pitch 1 after k0 scaling): k_d = 2 pi m_d / n_d
+    m = (1, 2, -3)
+    k = [2 * np.pi * m[d] / shape[d] for d in range(3)]
+    x = np.meshgrid(*[np.arange(n) for n in shape], indexing='ij')
+    phase = np.exp(1j * sum(k[d] * x[d] for d in range(3)))
+    a = np.array([0.3, -0.7 + 0.2j, 0.5j])
+    E = a[:, None, None, None, None] * phase[None, None]
+    expected = 1j * np.dot(k, a) * phase[None, None]
+    npt.assert_allclose(BE.asnumpy(BE.div(BE.astype(E))), expected, atol=2e-5 * np.abs(expected).max())
