@@ -102,6 +102,22 @@ def sol3d():
                        dtype=np.complex128)
 
 
+def test_grid_wavenumber_frequency_wavelength(sol3d):
+    """tests/test_solution.py:34-44, and the same medium specified by wavenumber or angular frequency."""
+    import macromax_amd as mm
+    wavelength = 4
+    assert sol3d.grid == mm.Grid(shape=[50, 100, 200], step=wavelength / 4)
+    npt.assert_almost_equal(sol3d.wavenumber, 2 * np.pi / wavelength)
+    npt.assert_almost_equal(sol3d.angular_frequency / (2 * np.pi * const.c / wavelength), 1.0)
+    npt.assert_almost_equal(sol3d.wavelength, wavelength)
+    grid = mm.Grid(shape=[16, 24], step=0.5)
+    for kw in ({'wavenumber': 2 * np.pi / wavelength}, {'angular_frequency': 2 * np.pi * const.c / wavelength},
+               {'vacuum_wavelength': wavelength}):
+        s = mm.Solution(grid, epsilon=1.5, current_density=np.ones([1, 16, 24]), **kw)
+        npt.assert_almost_equal(s.wavelength, wavelength, err_msg=str(kw))
+        npt.assert_almost_equal(s.wavenumber, 2 * np.pi / wavelength, err_msg=str(kw))
+
+
 def test_iteration_counter(sol3d):
     """tests/test_solution.py:46-57."""
     assert sol3d.iteration == 0
