@@ -357,14 +357,16 @@ int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, i
     // single-component strided passes; 8 columns of 3 components (16 of 1) for the Green pass
     int want = p_request > 0 ? p_request : (rows ? (NC == 3 ? 8 : 16) : (fused ? (NC == 3 ? 8 : 16) : 32));
     if (want > pmax) want = pmax;
-    // Small grids: more, smaller tiles fill more CUs, but single-component passes measured fastest with at least 4
-    // rows per tile and, for strided tiles, at least 16 columns (128-byte row segments) - e.g. a 512^2 3-component
-    // FFT 28 us with 1-row / 2-column tiles, 15 us with 8 / 16 (fused passes keep the smaller minimum).
-    const int rows_min = fused ? 1 : 4, cols_min = fused ? 2 : 16;
+    // Small grids: smaller tiles give more CUs work, down to 4 rows per tile, and to 16 columns (128-byte row
+    // segments) for single-component strided passes or 4 for fused ones. A sweep of 128^2-512^2 and 64^3 grids
+    // measured smaller tiles slower (a 512^2 3-component FFT: 28 us with 1-row / 2-column tiles, 15 us with 8 / 16;
+    // a 256^2 scalar Born iteration: 22.4 us with 1 / 2, 19.3 us with 4 / 4; a 64^3 vector one: 46.6 us with 4-row
+    // tiles, 42.4 us with 8).
+    const int rows_min = 4, cols_min = fused ? 4 : 16;
     if (rows) {
         const int64_t total_rows = t.nouter * mult;
         if (p_request <= 0) {
-            while (want > rows_min && total_rows / want < 1024) want >>= 1;  // keep >= ~1024 tiles when possible
+            while (want > rows_min && total_rows / want < 256) want >>= 1;  // keep >= ~256 tiles when possible
             while (want > 1 && want > t.nouter) want >>= 1;
         }
         t.P = pow2floor(want);
