@@ -252,8 +252,9 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
 }
 
-// ---- register-resident kernels (complex64, power-of-two N) ----
-// points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels
+// ---- register-resident kernels (complex64, N = 2^k or 3 * 2^k) ----
+// points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
+// factor 3 use 12 points per thread for three components (radix-4/2 stages, then radix 3) and 24 for one
 #ifndef MMX_E_FUSED3
 #define MMX_E_FUSED3 8
 #endif
@@ -261,15 +262,17 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 #define MMX_E_GREEN3 8
 #endif
 constexpr int reg_elems(int N, int NC, int KIND) {
-    return (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
+    return (N % 3 == 0)                              ? (NC == 3 ? 12 : 24)
+           : (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
            : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
            : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
 }
 
-// Power-of-two lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the
-// strided extent N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
+// Lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the strided extent
+// N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
 bool reg_supported(const AxisPlan& ax) {
-    const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512;
+    const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
+                     ax.N == 384;
     return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
 }
 
@@ -310,6 +313,9 @@ int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t 
         case 128: launch_reg<128, KIND, NC, CK>(t, a, stream); break;
         case 256: launch_reg<256, KIND, NC, CK>(t, a, stream); break;
         case 512: launch_reg<512, KIND, NC, CK>(t, a, stream); break;
+        case 96: launch_reg<96, KIND, NC, CK>(t, a, stream); break;
+        case 192: launch_reg<192, KIND, NC, CK>(t, a, stream); break;
+        case 384: launch_reg<384, KIND, NC, CK>(t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: register FFT length");
     }
     return MMX_OK;

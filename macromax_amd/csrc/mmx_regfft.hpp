@@ -1,4 +1,4 @@
-// Register-resident Stockham FFT for power-of-two axis lengths (compile-time N).
+// Register-resident Stockham FFT for axis lengths 2^k and 3 * 2^k (compile-time N).
 //
 // A pencil of N points is owned by TP = N/E threads; thread t holds the E points at positions t + m*TP
 // (m < E) in registers. Every stage of radix R (R divides E) runs E/R butterflies per thread on those registers;
@@ -43,21 +43,27 @@ template <typename T, int DIR> __device__ __forceinline__ void dft16(cx<T>* a) {
     }
 }
 
-template <typename T, int R, int DIR> __device__ __forceinline__ void dft_pow2(cx<T>* a) {
+template <typename T, int R, int DIR> __device__ __forceinline__ void dft_reg(cx<T>* a) {
     if constexpr (R == 2) dft2<T, DIR>(a);
+    else if constexpr (R == 3) dft3<T, DIR>(a);
     else if constexpr (R == 4) dft4<T, DIR>(a);
     else if constexpr (R == 8) dft8<T, DIR>(a);
     else dft16<T, DIR>(a);
 }
 
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
+constexpr int pow2_part(int x) { return x & -x; }
 
-// Radix of stage s for N = 2^n points and E registers per thread: the largest power of two <= min(E, 16)
-// that still divides what is left of N.
+// Radix of stage s for N = 2^n or 3 * 2^n points and E registers per thread (E a multiple of 3 when N is): the
+// power-of-two stages come first, each the largest power of two <= min(pow2 part of E, 16) that still divides what
+// is left of N's power-of-two part; a factor 3 of N is the last stage. The product of the radices before every
+// stage is then a power of two, so a butterfly's index within its group is i & (P - 1).
 constexpr int stage_radix(int N, int E, int s) {
-    int rem = N, r = 1;
+    const int cap = pow2_part(E) < 16 ? pow2_part(E) : 16;
+    int rem = pow2_part(N), r = 1;
     for (int q = 0; q <= s; ++q) {
-        r = E < 16 ? E : 16;
+        if (rem == 1) return N / pow2_part(N);
+        r = cap;
         while (r > rem) r >>= 1;
         rem /= r;
     }
@@ -75,7 +81,7 @@ constexpr int num_stages(int N, int E) {
 }
 // Twiddle table in LDS, laid out per stage: stage s >= 1 (radix R, P = product of earlier radices) keeps
 // w^(j k N/(P R)) for j = 1..R-1, k = 0..P-1 at tw_stage_off(s) + (j-1) P + k, so that the lanes of a wave, which
-// differ in k, read consecutive entries. The stages together hold N - 1 entries.
+// differ in k, read consecutive entries. The stages together hold at most N - 1 entries.
 constexpr int tw_stage_off(int N, int E, int s) {
     int off = 0;
     for (int q = 1; q < s; ++q) off += (stage_radix(N, E, q) - 1) * stage_prod(N, E, q);
@@ -104,10 +110,18 @@ __device__ __forceinline__ void load_twiddles(cx<T>* tw, const cx<T>* g, int tid
 // strides of the exchange. LDS has 64 four-byte banks; an 8-byte read serves 32 lanes per cycle and an 8-byte write
 // 16. With E = 8 (three radix-8/4 stages) a slot every 8 entries makes the strided-tile exchange and the rows-tile
 // writes conflict-free (rows-tile reads then pair up 3 of 32 lanes); with E = 16 a slot every 16 is conflict-free.
-constexpr int pad_shift(int E) { return E == 8 ? 3 : 4; }
+constexpr int pad_shift(int E) { return E == 8 ? 3 : 4; }  // 12 and 24 points per thread: a slot every 16
 template <int E> __device__ __forceinline__ int padpos(int pos) { return pos + (pos >> pad_shift(E)); }
 constexpr int padded_len(int N, int E) { return N + (N >> pad_shift(E)); }
 
+
+template <int N, int E> constexpr bool reg_schedule_ok() {
+    for (int q = 0; q < num_stages(N, E); ++q) {
+        const int R = stage_radix(N, E, q), P = stage_prod(N, E, q);
+        if (E % R != 0 || (P & (P - 1)) != 0) return false;
+    }
+    return N % E == 0 && pow2_part(N / E) == N / E;
+}
 
 // One stage on the registers v[NC][E] of all NC components.
 template <typename T, int N, int E, int S, int DIR, int NC>
@@ -137,7 +151,7 @@ __device__ __forceinline__ void reg_stage(cx<T> (&v)[NC][E], int t, const cx<T>*
 #pragma unroll
                 for (int j = 1; j < R; ++j) a[j] = a[j] * w[j];
             }
-            dft_pow2<T, R, DIR>(a);
+            dft_reg<T, R, DIR>(a);
 #pragma unroll
             for (int j = 0; j < R; ++j) v[c][q + j * BQ] = a[j];
         }
@@ -175,6 +189,7 @@ __device__ __forceinline__ void reg_exchange(cx<T> (&v)[NC][E], int t, cx<T>* xb
 // Whole FFT (unnormalised) of the NC pencils held in v, natural order in and out.
 template <typename T, int N, int E, int DIR, int NC, int S = 0>
 __device__ __forceinline__ void reg_fft(cx<T> (&v)[NC][E], int t, const cx<T>* tw, cx<T>* xb, int ps) {
+    static_assert(reg_schedule_ok<N, E>(), "every radix must divide E and the threads per pencil be a power of 2");
     reg_stage<T, N, E, S, DIR, NC>(v, t, tw);
     if constexpr (S + 1 < num_stages(N, E)) {
         reg_exchange<T, N, E, S, NC>(v, t, xb, ps);

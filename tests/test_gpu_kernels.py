@@ -23,7 +23,9 @@ def _plan(shape, step, dtype, device):
 
 
 FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96), (56, 40), (96, 96, 96),
-              (128, 128, 128), (1, 64, 1), (16, 1, 48), (3, 5, 7), (256, 256), (24, 2, 40)]
+              (128, 128, 128), (1, 64, 1), (16, 1, 48), (3, 5, 7), (256, 256), (24, 2, 40),
+              # lengths 3 * 2^k (register kernels with a radix-3 stage) on every axis position
+              (192,), (384,), (384, 192), (96, 384, 64), (64, 192, 96)]
 
 
 @pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
@@ -111,6 +113,9 @@ BORN_CASES = [
     ((32, 32, 32), 1, 1), ((24, 32, 40), 3, 1), ((16, 20, 24), 3, 2), ((5, 10, 20), 3, 2),
     ((14, 21, 8), 3, 1), ((8, 1, 12), 3, 2), ((64, 64, 64), 3, 2),
     ((16, 20, 24), 3, 3), ((64, 48), 3, 3), ((512,), 3, 3),
+    # lengths 3 * 2^k (register kernels with a radix-3 stage) with every chi kind
+    ((96, 96, 96), 3, 4), ((192, 384), 1, 1), ((384, 96), 3, 3), ((96, 192, 64), 3, 2), ((64, 96, 192), 3, 1),
+    ((384,), 3, 2), ((192, 64), 1, 1),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
     # uniaxial chi storage, LDS-staged and register kernels
@@ -214,7 +219,8 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
                                                 ((64, 64, 128), 1, 1, 'c64'), ((256, 64), 3, 2, 'c64'),
                                                 ((16, 20, 24), 3, 3, 'c128'), ((48, 64), 1, 1, 'c128'),
                                                 ((256, 256, 256), 3, 4, 'c64'), ((64, 128, 64), 3, 4, 'c64'),
-                                                ((16, 20, 24), 3, 4, 'c128')],
+                                                ((16, 20, 24), 3, 4, 'c128'), ((96, 96, 96), 3, 4, 'c64'),
+                                                ((192, 384), 1, 1, 'c64'), ((64, 96, 384), 3, 3, 'c64')],
                          ids=lambda v: str(v))
 def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
