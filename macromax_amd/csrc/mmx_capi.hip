@@ -254,15 +254,19 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 
 // ---- register-resident kernels (complex64, N = 2^k or 3 * 2^k) ----
 // points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
-// factor 3 use 12 points per thread for three components (radix-4/2 stages, then radix 3) and 24 for one
+// factor 3 use 6 points per thread for three components (radix-2 stages, then radix 3; 132-172 VGPRs, measured 2-6 %
+// faster than 12 points with 192-225 VGPRs) and 24 for one
 #ifndef MMX_E_FUSED3
 #define MMX_E_FUSED3 8
 #endif
 #ifndef MMX_E_GREEN3
 #define MMX_E_GREEN3 8
 #endif
+#ifndef MMX_E_RADIX3_C3
+#define MMX_E_RADIX3_C3 6
+#endif
 constexpr int reg_elems(int N, int NC, int KIND) {
-    return (N % 3 == 0)                              ? (NC == 3 ? 12 : 24)
+    return (N % 3 == 0)                              ? (NC == 3 ? MMX_E_RADIX3_C3 : 24)
            : (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
            : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
            : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
