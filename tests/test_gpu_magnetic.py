@@ -177,3 +177,52 @@ def test_magnetic_graph_replays_match_eager_launches(dtype):
         if it % 3 == 1:
             npt.assert_array_equal(graphed.H, eager.H)
     assert rejected >= 1
+
+
+def test_magnetic_graphs_follow_a_reallocated_plan_scratch():
+    """The plan's FP64 scratch buffer grows when a larger array is reduced on the same plan (MMX_PARAM_SCRATCH_GEN
+    counts the allocations); graphs recorded before hold the old address, so the next magnetic step records them
+    again, and the iterates stay equal to eager launches."""
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print('magnetic graphs after scratch growth', flush=True)
+    rng = np.random.default_rng(13)
+    shape = (32, 24)
+    grid = mm.Grid(list(shape), 0.1e-6)
+    mu = 1.0 + 0.3 * rng.random(shape)
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, 8, :] = 1.0
+    bound = LinearBound(grid, thickness=0.6e-6)
+
+    def make(graphs):
+        s = mm.Solution(grid, vacuum_wavelength=0.5e-6, current_density=J, epsilon=2.0, mu=mu, bound=bound,
+                        dtype=np.complex64)
+        s.magnetic_graphs = graphs
+        return s
+    eager, graphed = make(False), make(True)
+    for _ in range(4):
+        eager.step()
+        graphed.step()
+    plan = graphed._Solution__BE.plan
+    gen = plan.get_param('scratch_gen')
+    dev = graphed.field_array.device
+    V = int(np.prod(shape))
+    chi = torch.full((1, 1, *shape), 0.1, dtype=torch.complex64, device=dev)
+    s1 = torch.zeros((3, 1, *shape), dtype=torch.complex64, device=dev)
+    for nbatch in (64, 256, 1024):  # batched Born steps keep per-tile partial sums of every problem in the scratch
+        E = torch.ones((nbatch, 3, 1, *shape), dtype=torch.complex64, device=dev)
+        Eo = torch.empty_like(E)
+        coef = torch.tensor([1.0, 1.0, 1.0, 0.0, 0.0, 0.0] * nbatch, dtype=torch.float32, device=dev)
+        norms = torch.zeros(2 * nbatch, dtype=torch.float64, device=dev)
+        plan.born_step(E, Eo, s1, chi, 3, 1, coef, nbatch, 0, 0, norms)
+        torch.cuda.synchronize()
+        if plan.get_param('scratch_gen') > gen:
+            break
+    assert plan.get_param('scratch_gen') > gen, 'the scratch buffer was not reallocated'
+    assert np.all(np.isfinite(norms.cpu().numpy())) and V > 0
+    for it in range(4):
+        eager.step()
+        graphed.step()
+        npt.assert_array_equal(graphed.field_array.cpu().numpy(), eager.field_array.cpu().numpy(), err_msg=str(it))
+        assert graphed.residue == eager.residue
