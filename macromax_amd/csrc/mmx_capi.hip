@@ -9,7 +9,7 @@
 
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
-#include "mmx_reglaunch.hpp"
+#include "mmx_regkernels.hpp"
 #include "mmx_setup.hpp"
 
 using namespace mmx;
@@ -155,6 +155,17 @@ int pow2floor(int64_t x) {
     return r;
 }
 
+// Tile shape for one pass: pencils per tile P, threads, dynamic LDS bytes, tiles per problem.
+struct TileCfg {
+    int P, nth, pad;
+    int big;                    // 1: the kernel variant with the larger per-thread element budget
+    size_t lds;
+    int64_t nouter;             // outer blocks the pass iterates (folded passes include batch and components)
+    int64_t ntile_inner;
+    int64_t tiles_per_problem;
+    int64_t nblocks;
+};
+
 // fused: tiles hold `ncomp` components of one problem, grid = tiles_per_problem * nbatch.
 // plain: (batch, component, outer) folded into nouter, ncomp = 1, `nbatch` counts fields.
 template <typename T>
@@ -241,28 +252,103 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
 }
 
-// ---- register-resident kernels (complex64, N = 2^k or 3 * 2^k; launchers in mmx_reg_<N>.hip) ----
+// ---- register-resident kernels (complex64, N = 2^k or 3 * 2^k) ----
+// points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
+// factor 3 use 6 points per thread for three components (radix-2 stages, then radix 3; 132-172 VGPRs, measured 2-6 %
+// faster than 12 points with 192-225 VGPRs) and 24 for one
+#ifndef MMX_E_FUSED3
+#define MMX_E_FUSED3 8
+#endif
+#ifndef MMX_E_GREEN3
+#define MMX_E_GREEN3 8
+#endif
+#ifndef MMX_E_RADIX3_C3
+#define MMX_E_RADIX3_C3 6
+#endif
+constexpr int reg_elems(int N, int NC, int KIND) {
+    return (N % 3 == 0)                              ? (NC == 3 ? MMX_E_RADIX3_C3 : 24)
+           : (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
+           : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
+           : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
+}
+
 // Lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the strided extent
 // N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
 bool reg_supported(const AxisPlan& ax) {
-    bool len = false;
-#define MMX_REG_IS(L) len = len || ax.N == L;
-    MMX_REG_LENGTHS(MMX_REG_IS)
-#undef MMX_REG_IS
+    const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
+                     ax.N == 384;
     return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
 }
 
 int reg_threads_per_pencil(int N, int NC, int kind) { return N / reg_elems(N, NC, kind); }
 
-// The register kernels of one length live in their own translation unit.
-int launch_reg_any(int N, int kind, bool three, int chi_kind, const TileCfg& t, const PassArgs<float>& a,
-                   hipStream_t stream) {
+// Dynamic LDS beyond 64 KiB has to be enabled per kernel (up to the 160 KiB of a gfx950 CU).
+template <auto KERNEL>
+void allow_lds(size_t lds) {
+    if (lds <= 65536) return;
+    static std::atomic<size_t> raised{0};  // per kernel: the largest dynamic LDS size enabled so far
+    if (lds > raised.load()) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(KERNEL), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        raised.store(lds);
+    }
+}
+
+template <int N, int KIND, int NC, int CK>
+void launch_reg(const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
+    constexpr int E = reg_elems(N, NC, KIND);
+    if constexpr (KIND == kEpiPro || KIND == kGreenEpiPro) {
+        if (a.s_restricted) {
+            allow_lds<&reg_pass_kernel<float, N, E, KIND, NC, CK, 1>>(t.lds);
+            hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 1>), dim3((unsigned)t.nblocks),
+                               dim3(t.nth), t.lds, stream, a);
+            return;
+        }
+    }
+    allow_lds<&reg_pass_kernel<float, N, E, KIND, NC, CK, 0>>(t.lds);
+    hipLaunchKernelGGL((reg_pass_kernel<float, N, E, KIND, NC, CK, 0>), dim3((unsigned)t.nblocks), dim3(t.nth),
+                       t.lds, stream, a);
+}
+
+template <int KIND, int NC, int CK>
+int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
     switch (N) {
-#define MMX_REG_CASE(L) \
-    case L: return launch_reg_##L(kind, three, chi_kind, t, a, stream);
-        MMX_REG_LENGTHS(MMX_REG_CASE)
-#undef MMX_REG_CASE
-        default: return MMX_EINVAL;
+        case 64: launch_reg<64, KIND, NC, CK>(t, a, stream); break;
+        case 128: launch_reg<128, KIND, NC, CK>(t, a, stream); break;
+        case 256: launch_reg<256, KIND, NC, CK>(t, a, stream); break;
+        case 512: launch_reg<512, KIND, NC, CK>(t, a, stream); break;
+        case 96: launch_reg<96, KIND, NC, CK>(t, a, stream); break;
+        case 192: launch_reg<192, KIND, NC, CK>(t, a, stream); break;
+        case 384: launch_reg<384, KIND, NC, CK>(t, a, stream); break;
+        default: return fail(MMX_EINVAL, "internal: register FFT length");
+    }
+    return MMX_OK;
+}
+
+constexpr bool kind_applies_chi(int kind) {
+    return kind == kProFwd || kind == kInvEpi || kind == kAll || kind == kEpiPro || kind == kGreenEpiPro;
+}
+
+// Instantiation for the component count and (for passes that apply chi) the chi kind of the launch.
+template <int KIND>
+int launch_reg_kind(int N, bool three, int chi_kind, const TileCfg& t, const PassArgs<float>& a, hipStream_t stream) {
+    if constexpr (KIND == kFwd || KIND == kInv) {  // single-component passes
+        if (three) return fail(MMX_EINVAL, "internal: plain pass with three components");
+        return launch_reg_n<KIND, 1, 0>(N, t, a, stream);
+    } else if constexpr (!kind_applies_chi(KIND)) {
+        return three ? launch_reg_n<KIND, 3, 0>(N, t, a, stream) : launch_reg_n<KIND, 1, 0>(N, t, a, stream);
+    } else {
+        if (!three) {
+            if (chi_kind != 1) return fail(MMX_EINVAL, "internal: 3x3 chi with one component");
+            return launch_reg_n<KIND, 1, 1>(N, t, a, stream);
+        }
+        switch (chi_kind) {
+            case 1: return launch_reg_n<KIND, 3, 1>(N, t, a, stream);
+            case 2: return launch_reg_n<KIND, 3, 2>(N, t, a, stream);
+            case 3: return launch_reg_n<KIND, 3, 3>(N, t, a, stream);
+            case 4: return launch_reg_n<KIND, 3, 4>(N, t, a, stream);
+            default: return fail(MMX_EINVAL, "internal: chi kind");
+        }
     }
 }
 
@@ -355,8 +441,18 @@ int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, 
     if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
     const bool three = fused && a.ncomp == 3;
     const int ck = a.chi_kind;
-    if (launch_reg_any(ax.N, kind, three, ck, t, a, stream) != MMX_OK)
-        return fail(MMX_EINVAL, "internal: no register kernel for this length, pass kind and component count");
+    switch (kind) {
+        case kFwd: rc = launch_reg_kind<kFwd>(ax.N, three, ck, t, a, stream); break;
+        case kInv: rc = launch_reg_kind<kInv>(ax.N, three, ck, t, a, stream); break;
+        case kProFwd: rc = launch_reg_kind<kProFwd>(ax.N, three, ck, t, a, stream); break;
+        case kGreen: rc = launch_reg_kind<kGreen>(ax.N, three, ck, t, a, stream); break;
+        case kInvEpi: rc = launch_reg_kind<kInvEpi>(ax.N, three, ck, t, a, stream); break;
+        case kAll: rc = launch_reg_kind<kAll>(ax.N, three, ck, t, a, stream); break;
+        case kEpiPro: rc = launch_reg_kind<kEpiPro>(ax.N, three, ck, t, a, stream); break;
+        case kGreenEpiPro: rc = launch_reg_kind<kGreenEpiPro>(ax.N, three, ck, t, a, stream); break;
+        default: return fail(MMX_EINVAL, "internal: unknown pass kind");
+    }
+    if (rc) return rc;
     MMX_HIP_CHECK(hipGetLastError());
     return MMX_OK;
 }

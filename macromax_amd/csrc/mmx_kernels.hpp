@@ -480,8 +480,7 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
 }
 
 // Sum of per-tile partials -> out[2*b + {0,1}], fixed order (deterministic). One block per problem.
-// (Non-template kernels of this header have internal linkage: several translation units include it.)
-static __global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
+__global__ void __launch_bounds__(1024) reduce_partials_kernel(const double* partials, int64_t tiles, double* out) {
     __shared__ double red[16];
     const int64_t b = blockIdx.x;
     const double2* pp = reinterpret_cast<const double2*>(partials + 2 * b * tiles);
@@ -580,7 +579,7 @@ __global__ void __launch_bounds__(256) batch_control_kernel(int64_t nbatch, cons
     }
 }
 
-static __global__ void wall_clock_kernel(double* out) { out[0] = double(wall_clock64()); }
+__global__ void wall_clock_kernel(double* out) { out[0] = double(wall_clock64()); }
 
 // sum |x|^2 in FP64: per-block partials (pass 1), then reduce_partials-style sum (pass 2).
 template <typename T>
@@ -593,7 +592,7 @@ __global__ void __launch_bounds__(256) norm2_partial_kernel(const cx<T>* x, int6
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-static __global__ void __launch_bounds__(256) sum_kernel(const double* partials, int64_t n, double* out) {
+__global__ void __launch_bounds__(256) sum_kernel(const double* partials, int64_t n, double* out) {
     __shared__ double red[16];
     double s = 0.0;
     for (int64_t q = threadIdx.x; q < n; q += blockDim.x) s += partials[q];
