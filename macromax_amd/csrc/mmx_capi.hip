@@ -252,10 +252,10 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
 }
 
-// ---- register-resident kernels (complex64, N = 2^k or 3 * 2^k) ----
+// ---- register-resident kernels (complex64, N = 2^k, 3 * 2^k or 5 * 2^k) ----
 // points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
 // factor 3 use 6 points per thread for three components (radix-2 stages, then radix 3; 132-172 VGPRs, measured 2-6 %
-// faster than 12 points with 192-225 VGPRs) and 24 for one
+// faster than 12 points with 192-225 VGPRs) and 24 for one; lengths with a factor 5 use 10 and 20
 #ifndef MMX_E_FUSED3
 #define MMX_E_FUSED3 8
 #endif
@@ -266,7 +266,8 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
 #define MMX_E_RADIX3_C3 6
 #endif
 constexpr int reg_elems(int N, int NC, int KIND) {
-    return (N % 3 == 0)                              ? (NC == 3 ? MMX_E_RADIX3_C3 : 24)
+    return (N % 5 == 0)                              ? (NC == 3 ? 10 : 20)
+           : (N % 3 == 0)                              ? (NC == 3 ? MMX_E_RADIX3_C3 : 24)
            : (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
            : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
            : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
@@ -276,7 +277,7 @@ constexpr int reg_elems(int N, int NC, int KIND) {
 // N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
 bool reg_supported(const AxisPlan& ax) {
     const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
-                     ax.N == 384;
+                     ax.N == 384 || ax.N == 80 || ax.N == 160 || ax.N == 320;
     return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
 }
 
@@ -320,6 +321,9 @@ int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t 
         case 96: launch_reg<96, KIND, NC, CK>(t, a, stream); break;
         case 192: launch_reg<192, KIND, NC, CK>(t, a, stream); break;
         case 384: launch_reg<384, KIND, NC, CK>(t, a, stream); break;
+        case 80: launch_reg<80, KIND, NC, CK>(t, a, stream); break;
+        case 160: launch_reg<160, KIND, NC, CK>(t, a, stream); break;
+        case 320: launch_reg<320, KIND, NC, CK>(t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: register FFT length");
     }
     return MMX_OK;

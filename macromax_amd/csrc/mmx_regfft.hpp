@@ -1,4 +1,4 @@
-// Register-resident Stockham FFT for axis lengths 2^k and 3 * 2^k (compile-time N).
+// Register-resident Stockham FFT for axis lengths q * 2^k with q = 1, 3 or 5 (compile-time N).
 //
 // A pencil of N points is owned by TP = N/E threads; thread t holds the E points at positions t + m*TP
 // (m < E) in registers. Every stage of radix R (R divides E) runs E/R butterflies per thread on those registers;
@@ -47,6 +47,7 @@ template <typename T, int R, int DIR> __device__ __forceinline__ void dft_reg(cx
     if constexpr (R == 2) dft2<T, DIR>(a);
     else if constexpr (R == 3) dft3<T, DIR>(a);
     else if constexpr (R == 4) dft4<T, DIR>(a);
+    else if constexpr (R == 5) dft5<T, DIR>(a);
     else if constexpr (R == 8) dft8<T, DIR>(a);
     else dft16<T, DIR>(a);
 }
@@ -54,10 +55,11 @@ template <typename T, int R, int DIR> __device__ __forceinline__ void dft_reg(cx
 constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x >> 1); }
 constexpr int pow2_part(int x) { return x & -x; }
 
-// Radix of stage s for N = 2^n or 3 * 2^n points and E registers per thread (E a multiple of 3 when N is): the
-// power-of-two stages come first, each the largest power of two <= min(pow2 part of E, 16) that still divides what
-// is left of N's power-of-two part; a factor 3 of N is the last stage. The product of the radices before every
-// stage is then a power of two, so a butterfly's index within its group is i & (P - 1).
+// Radix of stage s for N = q * 2^n points with an odd factor q (3 or 5 here) and E registers per thread (E a
+// multiple of q with an even power-of-two part): the power-of-two stages come first, each the largest power of two
+// <= min(pow2 part of E, 16) that still divides what is left of N's power-of-two part; q is the last stage. The
+// product of the radices before every stage is then a power of two, so a butterfly's index within its group is
+// i & (P - 1).
 constexpr int stage_radix(int N, int E, int s) {
     const int cap = pow2_part(E) < 16 ? pow2_part(E) : 16;
     int rem = pow2_part(N), r = 1;

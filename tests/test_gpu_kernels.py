@@ -25,7 +25,9 @@ def _plan(shape, step, dtype, device):
 FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96), (56, 40), (96, 96, 96),
               (128, 128, 128), (1, 64, 1), (16, 1, 48), (3, 5, 7), (256, 256), (24, 2, 40),
               # lengths 3 * 2^k (register kernels with a radix-3 stage) on every axis position
-              (192,), (384,), (384, 192), (96, 384, 64), (64, 192, 96)]
+              (192,), (384,), (384, 192), (96, 384, 64), (64, 192, 96),
+              # lengths 5 * 2^k (register kernels with a radix-5 stage)
+              (320,), (160, 80), (80, 160, 64), (64, 320, 80)]
 
 
 @pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
@@ -116,6 +118,8 @@ BORN_CASES = [
     # lengths 3 * 2^k (register kernels with a radix-3 stage) with every chi kind
     ((96, 96, 96), 3, 4), ((192, 384), 1, 1), ((384, 96), 3, 3), ((96, 192, 64), 3, 2), ((64, 96, 192), 3, 1),
     ((384,), 3, 2), ((192, 64), 1, 1),
+    # lengths 5 * 2^k (register kernels with a radix-5 stage)
+    ((80, 80, 80), 3, 4), ((160, 320), 1, 1), ((320, 80), 3, 3), ((80, 160, 64), 3, 2), ((320,), 3, 1),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
     # uniaxial chi storage, LDS-staged and register kernels
@@ -220,7 +224,8 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
                                                 ((16, 20, 24), 3, 3, 'c128'), ((48, 64), 1, 1, 'c128'),
                                                 ((256, 256, 256), 3, 4, 'c64'), ((64, 128, 64), 3, 4, 'c64'),
                                                 ((16, 20, 24), 3, 4, 'c128'), ((96, 96, 96), 3, 4, 'c64'),
-                                                ((192, 384), 1, 1, 'c64'), ((64, 96, 384), 3, 3, 'c64')],
+                                                ((192, 384), 1, 1, 'c64'), ((64, 96, 384), 3, 3, 'c64'),
+                                                ((80, 80, 80), 3, 4, 'c64'), ((160, 320), 1, 1, 'c64')],
                          ids=lambda v: str(v))
 def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
