@@ -27,7 +27,9 @@ FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96),
               # lengths 3 * 2^k (register kernels with a radix-3 stage) on every axis position
               (192,), (384,), (384, 192), (96, 384, 64), (64, 192, 96),
               # lengths 5 * 2^k (register kernels with a radix-5 stage)
-              (320,), (160, 80), (80, 160, 64), (64, 320, 80)]
+              (320,), (160, 80), (80, 160, 64), (64, 320, 80),
+              # 640, 768 and 1024 points (register kernels with up to 128 threads per pencil)
+              (1024, 64), (768, 640), (64, 1024)]
 
 
 @pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
@@ -120,6 +122,8 @@ BORN_CASES = [
     ((384,), 3, 2), ((192, 64), 1, 1),
     # lengths 5 * 2^k (register kernels with a radix-5 stage)
     ((80, 80, 80), 3, 4), ((160, 320), 1, 1), ((320, 80), 3, 3), ((80, 160, 64), 3, 2), ((320,), 3, 1),
+    # 640, 768 and 1024 points
+    ((1024, 256), 3, 3), ((640, 768), 1, 1), ((768, 128), 3, 4), ((64, 1024), 3, 2),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
     # uniaxial chi storage, LDS-staged and register kernels
@@ -225,7 +229,8 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
                                                 ((256, 256, 256), 3, 4, 'c64'), ((64, 128, 64), 3, 4, 'c64'),
                                                 ((16, 20, 24), 3, 4, 'c128'), ((96, 96, 96), 3, 4, 'c64'),
                                                 ((192, 384), 1, 1, 'c64'), ((64, 96, 384), 3, 3, 'c64'),
-                                                ((80, 80, 80), 3, 4, 'c64'), ((160, 320), 1, 1, 'c64')],
+                                                ((80, 80, 80), 3, 4, 'c64'), ((160, 320), 1, 1, 'c64'),
+                                                ((1024, 256), 3, 3, 'c64'), ((768, 640), 1, 1, 'c64')],
                          ids=lambda v: str(v))
 def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
