@@ -252,7 +252,7 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
 }
 
-// ---- register-resident kernels (complex64, N = 2^k, 3 * 2^k or 5 * 2^k, 64 <= N <= 1024) ----
+// ---- register-resident kernels (complex64, N = 2^k, 3 * 2^k or 5 * 2^k, 64 <= N <= 2048) ----
 // points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
 // factor 3 use 6 points per thread for three components (radix-2 stages, then radix 3; 132-172 VGPRs, measured 2-6 %
 // faster than 12 points with 192-225 VGPRs) and 24 for one; lengths with a factor 5 use 10 and 20
@@ -278,7 +278,7 @@ constexpr int reg_elems(int N, int NC, int KIND) {
 bool reg_supported(const AxisPlan& ax) {
     const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
                      ax.N == 384 || ax.N == 80 || ax.N == 160 || ax.N == 320 || ax.N == 640 || ax.N == 768 ||
-                     ax.N == 1024;
+                     ax.N == 1024 || ax.N == 2048;
     return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
 }
 
@@ -328,6 +328,7 @@ int launch_reg_n(int N, const TileCfg& t, const PassArgs<float>& a, hipStream_t 
         case 640: launch_reg<640, KIND, NC, CK>(t, a, stream); break;
         case 768: launch_reg<768, KIND, NC, CK>(t, a, stream); break;
         case 1024: launch_reg<1024, KIND, NC, CK>(t, a, stream); break;
+        case 2048: launch_reg<2048, KIND, NC, CK>(t, a, stream); break;
         default: return fail(MMX_EINVAL, "internal: register FFT length");
     }
     return MMX_OK;

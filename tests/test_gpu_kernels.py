@@ -29,7 +29,7 @@ FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96),
               # lengths 5 * 2^k (register kernels with a radix-5 stage)
               (320,), (160, 80), (80, 160, 64), (64, 320, 80),
               # 640, 768 and 1024 points (register kernels with up to 128 threads per pencil)
-              (1024, 64), (768, 640), (64, 1024)]
+              (1024, 64), (768, 640), (64, 1024), (2048, 32), (16, 2048)]
 
 
 @pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
@@ -123,7 +123,8 @@ BORN_CASES = [
     # lengths 5 * 2^k (register kernels with a radix-5 stage)
     ((80, 80, 80), 3, 4), ((160, 320), 1, 1), ((320, 80), 3, 3), ((80, 160, 64), 3, 2), ((320,), 3, 1),
     # 640, 768 and 1024 points
-    ((1024, 256), 3, 3), ((640, 768), 1, 1), ((768, 128), 3, 4), ((64, 1024), 3, 2),
+    ((1024, 256), 3, 3), ((640, 768), 1, 1), ((768, 128), 3, 4), ((64, 1024), 3, 2), ((2048, 64), 3, 3),
+    ((64, 2048), 1, 1),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
     # uniaxial chi storage, LDS-staged and register kernels
