@@ -55,7 +55,8 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *   MMX_PARAM_LDS_BUDGET preferred LDS bytes per workgroup
  *   MMX_PARAM_BIG_ELEMS  bit mask over pass kinds (bit 0 forward, 1 inverse, 2 prologue+forward, 3 Green,
  *                        4 inverse+update, 5 single-axis): those passes give each thread twice the elements
- *   MMX_PARAM_REG_FFT    1 (default): complex64 axes of 64, 128, 256 or 512 points use the register-resident
+ *   MMX_PARAM_REG_FFT    1 (default): complex64 axes of 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640,
+ *                        768, 1024 or 2048 points use the register-resident
  *                        FFT kernels; 0: every axis uses the LDS-staged kernels
  *   MMX_PARAM_TRANSPOSED 0 (default): the operator of the reference (k = 2 pi fftfreq, utils/ft/grid.py:426-439);
  *                        1: its transpose, for adjoint solves by reciprocity: the Nyquist entry of every

@@ -1,4 +1,4 @@
-// Pencil passes for power-of-two axis lengths with the register-resident FFT (mmx_regfft.hpp).
+// Pencil passes with the register-resident FFT (mmx_regfft.hpp) for the axis lengths it is instantiated for.
 //
 // Same pass kinds and semantics as pencil_pass_kernel (mmx_kernels.hpp); the tile's data lives in registers:
 // thread (pencil p, position t) holds points t + m*TP (m < E) of each of its NC components.
