@@ -124,12 +124,18 @@ class MagneticOperator:
             return m * x
         return self.BE.mul(m, x.contiguous())
 
+    def _mul_add(self, m1, x1, m2, x2):
+        """m1 x1 + m2 x2; two element-wise kernels when both factors are scalar fields."""
+        if m1.shape[0] == 1 and m2.shape[0] == 1:
+            return torch.addcmul(m1 * x1, m2, x2)
+        return self._mul(m1, x1) + self._mul(m2, x2)
+
     def chi_op(self, E: torch.Tensor) -> torch.Tensor:
         """chi_EE E + chi_HE curl(E) + curl(chi_EH E + chi_HH curl(E)) (``solver.py:532-552``)."""
         BE = self.BE
         ED = BE.curl(E)
-        chiE = self._mul(self.chiEE, E) + self._mul(self.chiHE, ED)
-        chiH = self._mul(self.chiEH, E) + self._mul(self.chiHH, ED)
+        chiE = self._mul_add(self.chiEE, E, self.chiHE, ED)
+        chiH = self._mul_add(self.chiEH, E, self.chiHH, ED)
         return chiE + BE.curl(chiH)
 
     def green(self, X: torch.Tensor) -> torch.Tensor:
@@ -142,7 +148,7 @@ class MagneticOperator:
             self._g = (a, ((-1.0j * a.imag) / (BE.k2.to(torch.complex128) - a)).to(BE.hardware_dtype))
         g = self._g[1]
         PL = BE.longitudinal_projection_ft(F)
-        F = g * (F - PL) + PL * complex(1.0j * a.imag / a)
+        F = torch.addcmul(PL * complex(1.0j * a.imag / a), g, F - PL)  # g (F - PL) + (i alpha_i / alpha) PL
         return BE.ift(F)
 
     def update(self, E: torch.Tensor) -> torch.Tensor:
