@@ -428,26 +428,39 @@ def magnetic_benchmark(n=512, log=lambda m: None):
 # ---------------------------------------------------------------------------------------------------------------
 # Adjoint gradient: 96^3 isotropic vectorial, complex64; design cube 48^3 (eps uniform in [1, 2.25], seed 5),
 # y-polarised Gaussian current on plane z = 12, loss = sum |E|^2 over a 6^3 box centred 6 pixels behind the design.
-def adjoint_benchmark(n=96, design=48, tol=1e-6, max_iter=10000, n_fd=50, h=1e-3, log=lambda m: None):
+def adjoint_problem(n=96, design=48, bound_px=16, source_plane=12, seed=5):
+    """The adjoint workload: n^3 isotropic grid at 250 nm (1 um wavelength), a centred design cube of ``design``^3
+    voxels with eps uniform in [1, 2.25] (seeded), eps = 1 elsewhere, ``bound_px``-pixel LinearBound on all faces,
+    y-polarised Gaussian current (waist 2 um) on the plane z = ``source_plane``; the loss is sum |E|^2 over a 6^3
+    box centred 6 pixels behind the design region. Returns (grid, eps, J, bound, box, design slices, loss)."""
     import torch
     import macromax_amd as mm
-    from macromax_amd.adjoint import solve_field, fields_at_fixed_alpha
     from macromax_amd.bound import LinearBound
-    pitch, wavelength = 250e-9, 1e-6
+    pitch = 250e-9
     grid = mm.Grid(shape=[n, n, n], step=pitch)
-    bound = LinearBound(grid, thickness=16 * pitch)
+    bound = LinearBound(grid, thickness=bound_px * pitch)
     eps = np.ones((n, n, n))
     d0 = (n - design) // 2
     dsl = (slice(d0, d0 + design),) * 3
-    eps[dsl] = np.random.default_rng(5).uniform(1.0, 2.25, (design,) * 3)
+    eps[dsl] = np.random.default_rng(seed).uniform(1.0, 2.25, (design,) * 3)
     x = (np.arange(n) - n / 2) * pitch
     J = np.zeros((3, n, n, n), dtype=np.complex64)
-    J[1, :, :, 12] = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2e-6) ** 2)
+    J[1, :, :, source_plane] = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / (2e-6) ** 2)
     zc = d0 + design + 6
     box = (slice(n // 2 - 3, n // 2 + 3), slice(n // 2 - 3, n // 2 + 3), slice(zc - 3, zc + 3))
 
     def loss_of(E):
         return torch.sum(torch.abs(E[(slice(None),) + box]) ** 2)
+    return grid, eps, J, bound, box, dsl, loss_of
+
+
+def adjoint_benchmark(n=96, design=48, tol=1e-6, max_iter=10000, n_fd=50, h=1e-3, log=lambda m: None):
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.adjoint import solve_field, fields_at_fixed_alpha
+    wavelength = 1e-6
+    grid, eps, J, bound, box, dsl, loss_of = adjoint_problem(n, design)
+    d0 = dsl[0].start
 
     def gradient():
         eps_t = torch.tensor(eps, dtype=torch.float64, requires_grad=True)

@@ -192,11 +192,13 @@ class Solution:
         eps_full = eps64.expand(eps64.shape[0], eps64.shape[1], *full).contiguous()
         plan = BE.plan
         h_min, h_max = plan.setup_gain(eps_full)  # eigenvalues of -i/2 (eps - eps^H)
-        gain = h_min < -max_allowed_gain          # has_gain(eps), solver.py:322-327
-        loss = -h_max < -max_allowed_gain         # has_gain(-eps): the eigenvalues change sign
+        # the constant permeability counts too (solver.py:329-336); -i/2 (mu - mu^*) = Im(mu); xi = zeta = 0 here
+        mu_h = self.__mu.imag
+        gain = h_min < -max_allowed_gain or mu_h < -max_allowed_gain   # has_gain, solver.py:321-327
+        loss = -h_max < -max_allowed_gain or -mu_h < -max_allowed_gain  # has_gain(-a): the eigenvalues change sign
         transpose = gain and not loss
         if gain and not transpose:
-            log.warning('Convergence not guaranteed: the permittivity has gain.')
+            log.warning('Convergence not guaranteed: the permittivity or permeability has gain.')
 
         beta = float(np.real(1.0 / self.__mu))
 

@@ -15,7 +15,9 @@ oracle function                         reference
 ``linear_bound_extinction``             ``bound.py:159-183`` (AbsorbingBound.extinction), ``:196-215``
 ``bound_susceptibility``                ``bound.py:186-193``
 ``sigma_ee``                            ``solver.py:392-401`` (calc_DeltaEE_beta2 / calc_sigmaEE)
-``find_alpha_beta``                     ``solver.py:446-467`` (non-magnetic fmin) + ``:591-627`` (bias)
+``gain_eigenvalues`` / ``has_gain``      ``solver.py:321-327`` (eigenvalues of -i/2 (a - a^H))
+``find_alpha_beta``                     ``solver.py:329-336, 446-467`` (non-magnetic fmin, conjugate alpha for
+                                        gain-only media) + ``:591-627`` (bias)
 ``chi_base``                            ``solver.py:474-484``
 ``normalized_source``                   ``solver.py:702-720``
 ``longitudinal_projection_ft``          ``backend/__init__.py:606-649``
@@ -47,7 +49,8 @@ import scipy.optimize
 
 __all__ = [
     'k_vectors', 'linear_bound_extinction', 'bound_susceptibility', 'sigma_ee', 'find_alpha_beta',
-    'increase_bias', 'chi_base', 'normalized_source', 'longitudinal_projection_ft', 'green_op',
+    'increase_bias', 'gain_eigenvalues', 'has_gain', 'chi_base', 'normalized_source', 'longitudinal_projection_ft',
+    'green_op',
     'chi_mul', 'BornOracle', 'stockham_fft', 'plan_radices', 'mat3_max_eig_hermitian', 'setup_problem',
     'solve_to', 'adjoint_gradient', 'transposed_k_vectors', 'curl', 'magnetic_setup', 'MagneticBornOracle',
 ]
@@ -142,18 +145,44 @@ def increase_bias(alpha, bound_thickness_nonzero, wavenumber, max_kernel_field_r
     return alpha.real + 1j * offset
 
 
-def find_alpha_beta(eps, mu=1.0, bound_thickness=(), wavenumber=1.0):
+def gain_eigenvalues(a):
+    """Eigenvalues of the Hermitian part -i/2 (a - a^H) of a matrix field [n,n,...] (or a scalar), all voxels
+    (``solver.py:321-327``: a negative eigenvalue is gain, a positive one loss)."""
+    a = np.asarray(a, dtype=np.complex128)
+    if a.ndim < 2:
+        a = a.reshape(1, 1)
+    h = -0.5j * (a - np.conj(np.swapaxes(a, 0, 1)))
+    if h.shape[0] == 1:
+        return np.real(h[0, 0]).ravel()
+    m = np.moveaxis(h.reshape(h.shape[0], h.shape[1], -1), -1, 0)
+    return np.linalg.eigvalsh(m).ravel()
+
+
+def has_gain(a, max_allowed_gain):
+    """``solver.py:321-327``."""
+    return bool(np.any(gain_eigenvalues(a) < -max_allowed_gain))
+
+
+def find_alpha_beta(eps, mu=1.0, bound_thickness=(), wavenumber=1.0,
+                    max_allowed_gain=np.sqrt(np.finfo(np.float64).eps)):
     """Non-magnetic preconditioner constants (``solver.py:446-470``).
 
     beta = Re(1/mu); alpha_r = fmin(sigma_EE) with simplex [[0],[1]], xtol=ftol=0.01, maxiter=maxfun=100;
-    alpha = alpha_r + i sigma_min; conjugated for gain-only media (not handled: raises); then the bias.
+    alpha = alpha_r + i sigma_min, conjugated when the medium (eps and the constant mu) has gain and no loss
+    (``solver.py:329-336, 464-465``: gain and loss are eigenvalues of -i/2 (a - a^H) below -max_allowed_gain and
+    above +max_allowed_gain, the latter sqrt of the machine epsilon of the calculation's dtype); then the bias.
     """
     beta = float(np.real(1.0 / mu))
+    gain = has_gain(eps, max_allowed_gain) or has_gain(mu, max_allowed_gain)
+    loss = has_gain(-np.asarray(eps), max_allowed_gain) or has_gain(-np.asarray(mu), max_allowed_gain)
+    transpose = gain and not loss
     alpha_real, min_value = scipy.optimize.fmin(
         lambda a: sigma_ee(eps, float(np.asarray(a).ravel()[0]), beta), 0.0,
         initial_simplex=[[0.0], [1.0]], disp=False, full_output=True,
         ftol=0.01, xtol=0.01, maxiter=100, maxfun=100)[:2]
     alpha = alpha_real[0] + 1.0j * min_value
+    if transpose:
+        alpha = np.conj(alpha)
     alpha = increase_bias(alpha, bound_thickness, wavenumber)
     return complex(alpha), beta
 
@@ -502,7 +531,8 @@ class MagneticBornOracle:
 
 
 def setup_problem(shape, pitch, wavenumber, eps, current_density=None, source_distribution=None,
-                  bound_thickness=None, bound_kappa=0.25, vectorial=True, mu=1.0):
+                  bound_thickness=None, bound_kappa=0.25, vectorial=True, mu=1.0,
+                  max_allowed_gain=np.sqrt(np.finfo(np.float64).eps)):
     """Build (chi, s, alpha, beta, k, vectorial) from physical inputs as ``Solution.__init__`` does.
 
     ``solver.py:174-185`` (S = -i w mu0 J), ``:212`` (grid scaled by k0), ``:242-247`` (eps + bound chi),
@@ -528,7 +558,7 @@ def setup_problem(shape, pitch, wavenumber, eps, current_density=None, source_di
         if eps.shape[0] > 1:
             chi_b = np.eye(3).reshape(3, 3, *([1] * ndim)) * chi_b
         eps = eps + chi_b
-    alpha, beta = find_alpha_beta(eps, mu, thickness.ravel(), wavenumber)
+    alpha, beta = find_alpha_beta(eps, mu, thickness.ravel(), wavenumber, max_allowed_gain)
     chi = chi_base(np.broadcast_to(eps, (eps.shape[0], eps.shape[1], *shape)), alpha, beta)
     s = normalized_source(S, alpha, beta)
     k = k_vectors(shape, pitch * wavenumber)

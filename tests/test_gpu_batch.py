@@ -163,3 +163,41 @@ def test_batch_3d_anisotropic():
     for b, (s, r) in enumerate(zip(sols, ref)):
         assert s.iteration == r.iteration, b
         assert _rel(s.E, r.E) < 1e-5, b
+
+
+def test_batch_controller_matches_oracle():
+    """The device-side accept/reject rule of mmx_batch_step (batch_control_kernel) against the oracle directly:
+    per problem and per iteration the accept flag, alpha and the field of BornOracle.step (solver.py:1064-1098), on
+    under-damped problems that reject updates."""
+    from macromax_amd.batch import BatchSolution
+    from oracle import born_oracle as ob
+    print('batch controller against the oracle, 20 iterations', flush=True)
+    shape = (48, 64)
+    sols = []
+    for seed in range(3):
+        s = _scalar2d(shape, 40 + seed, np.complex128)
+        s._Solution__update_operators(complex(s.alpha.real, s.alpha.imag * (0.02 + 0.01 * seed)))
+        sols.append(s)
+    oracles = []
+    for s in sols:
+        k = ob.k_vectors(shape, np.asarray(s.grid.step) * s.wavenumber)
+        oracles.append(ob.BornOracle(s.chi_effective.cpu().numpy(), s.source_normalized.cpu().numpy(), s.alpha,
+                                     s.beta, k, False))
+    bt = BatchSolution(sols)
+    prev_rej = np.zeros(3, dtype=np.int64)
+    n_rej = 0
+    for it in range(1, 21):
+        bt.step()
+        rej = bt.rejected
+        alpha = bt.alpha
+        E = bt.field_arrays().cpu().numpy()
+        assert np.all(bt.iteration == it)
+        for b, o in enumerate(oracles):
+            acc = o.step()
+            assert acc == (rej[b] == prev_rej[b]), (it, b)
+            assert alpha[b] == pytest.approx(o.alpha, rel=1e-12), (it, b)
+            assert _rel(E[b], o.E) < 1e-10, (it, b, _rel(E[b], o.E))
+        n_rej += int(np.sum(rej - prev_rej))
+        prev_rej = rej
+    assert n_rej >= 2
+    np.testing.assert_allclose(bt.residue, [o.relative_residue for o in oracles], rtol=1e-9)

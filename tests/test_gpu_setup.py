@@ -73,3 +73,43 @@ def test_solution_alpha_matches_oracle(kind):
                                               bound_thickness=0.8e-6)
     np.testing.assert_allclose(sol.alpha.real, alpha.real, rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(sol.alpha.imag, alpha.imag, rtol=1e-9)
+
+
+@pytest.mark.parametrize('case', ['eps_gain', 'eps_gain_mu_loss', 'mu_gain', 'eps_gain_c64'])
+def test_gain_only_media_alpha_and_iterates_match_oracle(case):
+    """Gain-only media (no absorbing bound, whose extinction is loss): alpha is conjugated before the bias
+    (solver.py:329-336, 464-465) when eps or the constant mu has gain and neither has loss; 10 iterates (with the
+    divergence guard) follow the oracle."""
+    import torch
+    import macromax_amd as mm
+    print(f'gain-only medium {case}', flush=True)
+    shape = (40, 48)
+    rng = np.random.default_rng(21)
+    n = 1.0 + 0.4 * rng.random(shape)
+    gain = 0.15 * rng.random(shape)
+    eps = n ** 2 - (0.0 if case == 'mu_gain' else 1j) * gain
+    mu = {'eps_gain': 1.0, 'eps_gain_c64': 1.0, 'eps_gain_mu_loss': 1.0 + 0.01j, 'mu_gain': 1.0 - 0.01j}[case]
+    dtype = np.complex64 if case.endswith('c64') else np.complex128
+    thr = float(np.sqrt(np.finfo(np.float32 if dtype == np.complex64 else np.float64).eps))
+    grid = mm.Grid(shape=list(shape), step=0.1e-6)
+    J = np.zeros((1, *shape), dtype=np.complex128)
+    J[0, 5, 10:38] = 1.0
+    sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, mu=mu, current_density=J.astype(dtype),
+                      dtype=dtype)
+    chi, s, alpha, beta, k = ob.setup_problem(shape, 0.1e-6, 2 * np.pi / 0.5e-6, eps[None, None],
+                                              current_density=J, vectorial=False, mu=mu, max_allowed_gain=thr)
+    conj = case != 'eps_gain_mu_loss'
+    assert (alpha.imag - 0.05 < 0) == conj  # sigma_min > 0.05 here, so the conjugate stays negative
+    np.testing.assert_allclose(sol.alpha.real, alpha.real, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(sol.alpha.imag, alpha.imag, rtol=1e-9)
+    np.testing.assert_allclose(sol.beta, beta, rtol=1e-12)
+    o = ob.BornOracle(chi, s, alpha, beta, k, False)
+    tol = 1e-5 if dtype == np.complex64 else 1e-10
+    for it in range(10):
+        acc = sol.step()
+        acc_o = o.step()
+        assert acc == acc_o, it
+        assert sol.alpha == pytest.approx(o.alpha, rel=1e-9), it
+        E = sol.field_array.cpu().numpy()
+        err = np.linalg.norm((E - o.E).ravel()) / max(np.linalg.norm(o.E.ravel()), 1e-300)
+        assert err < tol, (it, err)

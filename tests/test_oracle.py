@@ -8,7 +8,8 @@ Each oracle function is checked against something other than its own formula:
 - stockham_fft (the GPU pass schedule): brute-force DFT and numpy.fft; a planted twiddle-sign mistake fails;
 - green_op: single Fourier modes land on the right k bin with the 1/N of the inverse transform;
 - chi_mul: per-voxel matrix products written out by hand;
-- find_alpha_beta / increase_bias: brute-force scan of sigma_EE and the closed-form bias;
+- find_alpha_beta / increase_bias: brute-force scan of sigma_EE and the closed-form bias; the gain-only
+  conjugation of alpha against the same scan on gain, loss and mixed media;
 - mat3_max_eig_hermitian: numpy's general (non-Hermitian) eigen-solver;
 - BornOracle (iteration, accept/reject, residue): the analytic plane-wave solutions of tests/test_solution.py:91-358
   (iteration counts <= 70 / <= 85 to residue 1e-6, decimal=3 field error), the bookkeeping definitions of
@@ -209,6 +210,41 @@ def test_alpha_against_brute_force_scan_and_bias():
     npt.assert_allclose(a2.imag, 2 * np.sqrt(1.0 + kmin ** 2) * kmin)
     a3 = ob.increase_bias(complex(1.0, 5.0), [20.0], wavenumber=1.0)  # thick bound: the +0.05 margin wins
     npt.assert_allclose(a3.imag, 5.05)
+
+
+def test_gain_only_media_conjugate_alpha():
+    """solver.py:329-336, 464-465: alpha = conj(alpha_r + i sigma_min) when eps or the constant mu has gain and
+    neither has loss; media with both, or with loss only, keep alpha_r + i sigma_min. Pinned by a brute-force scan
+    of sigma_EE(alpha_r) = max |eps - alpha_r| and the +0.05 bias of a periodic grid."""
+    rng = np.random.default_rng(15)
+    r1, r2 = rng.random((1, 1, 40)), rng.random((1, 1, 40))
+    scan = np.linspace(-1, 4, 50001)
+
+    def brute(eps):
+        sig = np.array([np.max(np.abs(eps - a)) for a in scan])
+        return scan[np.argmin(sig)], sig.min()
+    thr = np.sqrt(np.finfo(np.float32).eps)
+    cases = [  # eps, mu, conjugated?
+        (1.0 + 1.5 * r1 - 0.2j * r2, 1.0, True),            # gain only
+        (1.0 + 1.5 * r1 + 0.2j * (r2 - 0.5), 1.0, False),   # gain and loss
+        (1.0 + 1.5 * r1 + 0.2j * r2, 1.0, False),           # loss only
+        (1.0 + 1.5 * r1 - 0.2j * r2, 1.0 + 0.01j, False),   # gain in eps, loss in mu
+        (1.0 + 1.5 * r1 + 0j, 1.0 - 0.01j, True),           # gain in mu only
+        (1.0 + 1.5 * r1 - 0.1 * thr * 1j * r2, 1.0, False),  # gain below the threshold counts as none
+    ]
+    for eps, mu, conj in cases:
+        alpha, beta = ob.find_alpha_beta(eps, mu, max_allowed_gain=thr)
+        # sigma_EE with beta = Re(1/mu): max |eps - alpha_r beta| / beta
+        a_best, sig_min = brute(eps / beta)
+        assert abs(alpha.real - a_best) < 0.02, (mu, alpha)
+        expected_imag = (-sig_min if conj else sig_min) + 0.05
+        assert abs(alpha.imag - expected_imag) < 0.01 * sig_min + 1e-3, (mu, conj, alpha, sig_min)
+    # the eigenvalue test of a 3x3 field: -i/2 (a - a^H) of a diagonal eps is Im(eps)
+    e3 = np.zeros((3, 3, 4), complex)
+    for i in range(3):
+        e3[i, i] = 2.0 - 0.1j * (i + 1)
+    npt.assert_allclose(np.sort(ob.gain_eigenvalues(e3)), np.sort(np.repeat([-0.1, -0.2, -0.3], 4)), atol=1e-14)
+    assert ob.has_gain(e3, 1e-3) and not ob.has_gain(-e3, 1e-3)
 
 
 def _plane_wave_problem(kind):
