@@ -121,6 +121,22 @@ int mmx_born_step(mmx_plan_t plan, const void* E, void* Eout, const void* s, con
                   double* norms, void* stream);
 
 /*
+ * mmx_born_step without any device allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as mmx_born_step (which allocates or grows the plan's scratch buffer for its
+ * per-tile FP64 norm partials on first use). Here the partials go to a static device array of
+ * MMX_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on until the process ends; a launch that
+ * needs more (2 doubles per tile of the last pass per problem: e.g. 128 KiB per problem at 256^3 with three
+ * components) fails with MMX_EINVAL before anything is launched. The array needs no clearing: every launch writes
+ * each partial it reads. All calls on one device share that device's array, so calls that overlap on one device
+ * (other streams, other host threads) race. The plan's device must be current.
+ */
+#define MMX_NOALLOC_SCRATCH_BYTES (4u << 20)
+int mmx_born_step_noalloc(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* chi, int ncomp,
+                          int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                          double* norms, void* stream);
+
+/*
  * The same iteration split so that consecutive iterations share one pass over the field (the last pass of
  * iteration n also computes the first pass of iteration n+1, reading chi, E and s once). W is a work buffer of the
  * size of E, distinct from E and Eout.

@@ -17,7 +17,8 @@ MMX_C64 = 0
 MMX_C128 = 1
 
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
-            'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_prologue',
+            'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_step_noalloc',
+            'mmx_born_prologue',
             'mmx_born_iterate',
             'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_setup_sigma', 'mmx_setup_gain',
             'mmx_norm2', 'mmx_chi_mul',
@@ -25,6 +26,8 @@ EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destro
 
 # batched-solve state bits and per-problem track record (include/macromax_hip.h, MMX_STATE_* / MMX_TRACK_*)
 STATE_PARITY, STATE_DONE, STATE_STALE = 1, 2, 4
+# size of the static device array that holds the norm partials of mmx_born_step_noalloc (MMX_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 4 << 20
 TRACK = {'prev': 0, 'field_norm': 1, 'residue': 2, 'iter': 3, 'alpha_r': 4, 'alpha_i': 5, 'a': 6, 'c_r': 7,
          'c_i': 8, 'done_clock': 9, 'rejected': 10}
 TRACK_LEN = 12
@@ -61,6 +64,8 @@ def _declare(L):
     L.mmx_fft.argtypes = [c_p, c_p, c_p, c_i64, c_int, c_p]
     L.mmx_born_step.restype = c_int
     L.mmx_born_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
+    L.mmx_born_step_noalloc.restype = c_int
+    L.mmx_born_step_noalloc.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
     L.mmx_born_prologue.restype = c_int
     L.mmx_born_prologue.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p]
     L.mmx_born_iterate.restype = c_int

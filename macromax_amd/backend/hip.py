@@ -137,7 +137,9 @@ class Plan:
         return out
 
     def born_step(self, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
-                  s_box=None):
+                  s_box=None, noalloc=False):
+        """One Born iteration (``mmx_born_step``; ``noalloc``: ``mmx_born_step_noalloc``, which keeps its norm
+        partials in the library's static device array and allocates nothing)."""
         for t, w in ((E, 'E'), (Eout, 'Eout'), (s, 's'), (chi, 'chi')):
             self._check(t, w)
         if coef.dtype != self.real_dtype or coef.device != self.device or coef.numel() < 6 * nbatch:
@@ -152,10 +154,12 @@ class Plan:
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
             raise ValueError('born_step: s size mismatch')
         self.set_source_support(*(s_box if s_box is not None else (None, None)))
-        _native.check(self._lib.mmx_born_step(self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(),
-                                              chi.data_ptr(), int(ncomp), int(chi_kind), coef.data_ptr(),
-                                              int(nbatch), int(s_bstride), int(chi_bstride), norms.data_ptr(),
-                                              _stream_ptr(self.device)), 'mmx_born_step')
+        fn, name = ((self._lib.mmx_born_step_noalloc, 'mmx_born_step_noalloc') if noalloc else
+                    (self._lib.mmx_born_step, 'mmx_born_step'))
+        with torch.cuda.device(self.device):
+            _native.check(fn(self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(), chi.data_ptr(), int(ncomp),
+                             int(chi_kind), coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
+                             norms.data_ptr(), _stream_ptr(self.device)), name)
         return norms
 
     def _born_checks(self, E, others, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride):

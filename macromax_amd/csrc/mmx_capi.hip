@@ -1,5 +1,6 @@
 // C ABI of libmacromax_hip.so: plans, pass scheduling and launches (declarations in include/macromax_hip.h).
 #include <atomic>
+#include <mutex>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -13,6 +14,11 @@
 #include "mmx_setup.hpp"
 
 using namespace mmx;
+
+// Norm partials of mmx_born_step_noalloc: one array per device, held until the process ends. Host code never uses
+// this name as an address (it is a host-side stand-in); it uses the device address that hipGetSymbolAddress
+// returns for the current device (noalloc_scratch below).
+static __device__ __align__(256) double scratch[MMX_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
@@ -684,10 +690,12 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
     return reduce_norms(partials, tiles_last, nbatch, norms, stream);
 }
 
+// fixed_partials != nullptr: the per-tile partial sums go there (at most fixed_cap bytes) instead of the plan's
+// scratch buffer, and nothing is allocated.
 template <typename T>
 int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* chi, int C, int chi_kind,
               const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride, double* norms,
-              hipStream_t stream) {
+              hipStream_t stream, double* fixed_partials = nullptr, size_t fixed_cap = 0) {
     if (C != 1 && C != 3) return fail(MMX_EINVAL, "ncomp must be 1 or 3");
     if (chi_kind < 1 || chi_kind > 4) return fail(MMX_EINVAL, "chi_kind must be 1, 2, 3 or 4");
     if (chi_kind >= 2 && C != 3) return fail(MMX_EINVAL, "3x3 chi needs 3 components");
@@ -700,9 +708,20 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
     int rc = last_pass_tiles<T>(p, inner, C, nbatch, d == 1 ? kAll : kInvEpi, tl);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
-    rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
-    if (rc) return rc;
-    double* partials = reinterpret_cast<double*>(p->scratch);
+    const size_t partial_bytes = (size_t)(2 * tiles_last * nbatch) * sizeof(double);
+    double* partials;
+    if (fixed_partials != nullptr) {
+        if (partial_bytes > fixed_cap)
+            return fail(MMX_EINVAL, "launch needs " + std::to_string(partial_bytes) +
+                                        " bytes of norm partials, more than the " + std::to_string(fixed_cap) +
+                                        " bytes of the static scratch array (fewer problems per call, or "
+                                        "mmx_born_step)");
+        partials = fixed_partials;
+    } else {
+        rc = ensure_scratch(p, partial_bytes);
+        if (rc) return rc;
+        partials = reinterpret_cast<double*>(p->scratch);
+    }
     PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
     cx<T>* W = reinterpret_cast<cx<T>*>(Eout);
 
@@ -792,6 +811,27 @@ int batch_step_impl(mmx_plan_s* p, void* EA, void* EB, void* W, const void* s, c
     hipLaunchKernelGGL((batch_control_kernel<T>), dim3(nblk), dim3(256), 0, stream, nbatch, norms, state,
                        reinterpret_cast<Coef<T>*>(coef), track, tol, double(max_iter));
     MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+// Device address of `scratch` on the plan's device, looked up once per device (with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(const mmx_plan_s* p, double** out) {
+    int cur = -1;
+    MMX_HIP_CHECK(hipGetDevice(&cur));
+    if (cur != p->device) return fail(MMX_EINVAL, "the plan's device is not the current device");
+    if (cur < 0 || cur >= kMaxDevices) return fail(MMX_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_ptr[cur] == nullptr) {
+        void* ptr = nullptr;
+        MMX_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) return fail(MMX_EHIP, "hipGetSymbolAddress returned no address for the scratch array");
+        g_noalloc_ptr[cur] = ptr;
+    }
+    *out = reinterpret_cast<double*>(g_noalloc_ptr[cur]);
     return MMX_OK;
 }
 
@@ -1003,6 +1043,24 @@ int mmx_born_step(mmx_plan_t p, const void* E, void* Eout, const void* s, const 
     return p->dtype == MMX_C64
                ? born_impl<float>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
                : born_impl<double>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
+}
+
+int mmx_born_step_noalloc(mmx_plan_t p, const void* E, void* Eout, const void* s, const void* chi, int ncomp,
+                          int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
+                          double* norms, void* stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !s || !chi || !coef || !norms) return fail(MMX_EINVAL, "null argument");
+    if (E == Eout) return fail(MMX_EINVAL, "E and Eout must be different buffers");
+    double* partials = nullptr;
+    int rc = noalloc_scratch(p, &partials);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const size_t cap = MMX_NOALLOC_SCRATCH_BYTES;
+    return p->dtype == MMX_C64
+               ? born_impl<float>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st,
+                                  partials, cap)
+               : born_impl<double>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
+                                   st, partials, cap);
 }
 
 int mmx_born_prologue(mmx_plan_t p, const void* E, void* W, const void* s, const void* chi, int ncomp, int chi_kind,
