@@ -96,7 +96,9 @@ def test_gain_only_media_alpha_and_iterates_match_oracle(case):
     J[0, 5, 10:38] = 1.0
     sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, mu=mu, current_density=J.astype(dtype),
                       dtype=dtype)
-    chi, s, alpha, beta, k = ob.setup_problem(shape, 0.1e-6, 2 * np.pi / 0.5e-6, eps[None, None],
+    # the Solution holds eps in its element type (BackEnd.astype) before the FP64 set-up
+    eps_held = eps.astype(dtype).astype(np.complex128)
+    chi, s, alpha, beta, k = ob.setup_problem(shape, 0.1e-6, 2 * np.pi / 0.5e-6, eps_held[None, None],
                                               current_density=J, vectorial=False, mu=mu, max_allowed_gain=thr)
     conj = case != 'eps_gain_mu_loss'
     assert (alpha.imag - 0.05 < 0) == conj  # sigma_min > 0.05 here, so the conjugate stays negative
