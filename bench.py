@@ -7,9 +7,14 @@ LinearBound on z (periodic x, y); complex64 fields, FP64 norms.
 
 A "step" is one Born iteration (Solution.step: the fused HIP kernel sequence plus the accept/reject decision).
 `value` is voxel-iterations/s with all inputs resident in HBM. The JSON line also carries the HBM roofline of the
-iteration (algorithmic bytes per voxel-iteration, 480 B for this workload; `traffic` is the HBM bytes per iteration
-measured with rocprofv3 counters, from profiles/born_iterate_256cubed_aniso.md) and a CPU baseline: the NumPy
-complex128 oracle iteration on this host (single-threaded numpy.fft), timed for one iteration of the same grid.
+iteration: `achieved` = the bytes the fused four-pass schedule must move per voxel-iteration (268 B with the
+uniaxial chi storage of this medium) x voxels / the iteration's HIP-event time; `traffic` is the HBM bytes per
+iteration measured with rocprofv3 counters (profiles/born_iterate_256cubed_aniso.md); the reference operator
+sequence's 480 B per voxel-iteration gives `alg_equivalent_frac`, which can exceed 1 because the schedule moves
+fewer bytes. The CPU baseline is the NumPy complex128 oracle iteration on this host (numpy.fft, one core; min of 3
+trials of 3 iterations), with a labelled scipy.fft multi-threaded variant beside it. Also reported: time to
+residual 1e-5, time to 1e-4 relative L2 against a complex128 solve (FP64 refinement), the same iteration with the
+symmetric and full chi storage, batched solves, the adjoint gradient, magnetic media and the other configurations.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--grid N]
 """
@@ -55,7 +60,7 @@ def build_anisotropic_medium(n, device, seed=2, n_seeds=2000, no=1.50, ne=1.65, 
     return eps
 
 
-def setup_solution(n, device, dtype=np.complex64):
+def setup_solution(n, device, dtype=np.complex64, chi_storage='auto'):
     import torch
     import macromax_amd as mm
     from macromax_amd.bound import LinearBound
@@ -70,33 +75,108 @@ def setup_solution(n, device, dtype=np.complex64):
     bound = LinearBound(grid, thickness=[[0, 0], [0, 0], [16 * pitch, 16 * pitch]])
     t0 = time.perf_counter()
     sol = mm.Solution(grid, vacuum_wavelength=wavelength, epsilon=eps, current_density=J, bound=bound,
-                      dtype=dtype)
+                      dtype=dtype, chi_storage=chi_storage)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
     del eps, J
     return sol, setup_s
 
 
-def cpu_baseline(sol, n):
-    """Complex128 NumPy oracle iterations of the same problem on this host (numpy.fft is single-threaded): a bounded
-    sample of at least 10 s of CPU work (whole iterations, at least 2, at most 8), reported per iteration."""
+def cpu_baseline(sol, n, trials=3, k=3):
+    """Complex128 NumPy oracle iterations of the same problem on this host (the reference NumPy backend's operator
+    sequence with numpy.fft, which runs on one core): one warm-up iteration, then the minimum over ``trials`` trials
+    of ``k`` iterations each (about 30 s of CPU work at 256^3), reported per iteration. Beside it, labelled
+    "tuned", the same iteration with scipy.fft on all cores of this process's affinity (the FFTs only; the
+    susceptibility products stay single-threaded NumPy), one warm-up and ``k`` iterations."""
     from oracle import born_oracle as ob
     chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
     s = sol.source_normalized.cpu().numpy().astype(np.complex128)
     E = sol.field_array.cpu().numpy().astype(np.complex128)
-    k = ob.k_vectors((n, n, n), np.asarray(sol.grid.step) * sol.wavenumber)
-    o = ob.BornOracle(chi, s, sol.alpha, sol.beta, k, True, E0=E)
+    k_vec = ob.k_vectors((n, n, n), np.asarray(sol.grid.step) * sol.wavenumber)
+    o = ob.BornOracle(chi, s, sol.alpha, sol.beta, k_vec, True, E0=E)
     t0 = time.perf_counter()
-    iters = 0
-    while iters < 8 and (iters < 2 or time.perf_counter() - t0 < 10.0):
-        o.step()
-        iters += 1
+    o.step()
     total = time.perf_counter() - t0
-    dt = total / iters
-    return {'value': n ** 3 / dt, 'unit': 'voxel-iter/s', 'cores': 1, 'kind': 'port',
-            'sample': f'{iters} complex128 iterations of the same {n}^3 anisotropic problem (oracle/born_oracle.py, '
-                      f'numpy.fft), {total:.1f} s', 'seconds_per_iteration': dt,
-            'os_cpu_count': os.cpu_count(), 'affinity_cores': len(os.sched_getaffinity(0))}
+    best = math.inf
+    for _ in range(trials):
+        t0 = time.perf_counter()
+        for _ in range(k):
+            o.step()
+        dt = (time.perf_counter() - t0) / k
+        total += dt * k
+        best = min(best, dt)
+    cores = len(os.sched_getaffinity(0))
+    ot = ob.BornOracle(chi, s, sol.alpha, sol.beta, k_vec, True, E0=E, workers=cores)
+    ot.step()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        ot.step()
+    dt_tuned = (time.perf_counter() - t0) / k
+    return {'value': n ** 3 / best, 'unit': 'voxel-iter/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{n}^3 anisotropic problem of the benchmark, complex128 oracle (oracle/born_oracle.py, numpy.fft): '
+                      f'1 warm-up iteration, then the minimum over {trials} trials of {k} iterations; '
+                      f'{total:.1f} s of CPU work', 'seconds_per_iteration': best,
+            'tuned': {'value': n ** 3 / dt_tuned, 'unit': 'voxel-iter/s', 'cores': cores,
+                      'seconds_per_iteration': dt_tuned,
+                      'what': f'the same iteration with scipy.fft(workers={cores}); not the reference\'s backend'},
+            'os_cpu_count': os.cpu_count(), 'affinity_cores': cores}
+
+
+def must_move_bytes(ndim, ncomp, elem, chi_kind, s_fraction=1.0):
+    """HBM bytes per voxel-iteration that the fused pass schedule must move (mmx_born_iterate): 2 ndim - 3 passes
+    that read and write the work field (C e each way), then the update pass, which reads the work field, E and chi
+    (and s on its support), and writes E' and the next work field. chi: e (isotropic), 9 e (full 3x3), 6 e
+    (symmetric), 3.5 e (uniaxial: three complex planes and one real plane)."""
+    ce = ncomp * elem
+    b_chi = {1: 1.0, 2: 9.0, 3: 6.0, 4: 3.5}[chi_kind] * elem
+    return (2 * ndim - 3) * 2 * ce + 4 * ce + b_chi + s_fraction * ce
+
+
+def algorithmic_bytes(ndim, ncomp, elem, chi_elems):
+    """The reference operator sequence's bytes per voxel-iteration: 4 d C e + 2 B_chi + B_s + C e (d forward and d
+    inverse axis passes over the field, chi read twice, s and E once), B_chi = chi_elems x e."""
+    return 4 * ndim * ncomp * elem + 2 * chi_elems * elem + 2 * ncomp * elem
+
+
+def time_iterations(sol, steps, warmup):
+    """Per-iteration time of Solution.step: wall clock and HIP events on the launch stream."""
+    import torch
+    for _ in range(warmup):
+        sol.step()
+    dev = sol.field_array.device
+    stream = torch.cuda.current_stream(dev)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        sol.step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall / steps, ev0.elapsed_time(ev1) * 1e-3 / steps
+
+
+def storage_benchmark(n, dev, steps=50, warmup=5, log=lambda m: None):
+    """The benchmark medium with chi stored as the full 3x3 (72 B/voxel in complex64) and as the 6 symmetric entries
+    (48 B) instead of the uniaxial form (28 B): per-iteration time and bandwidth of the same iteration."""
+    import torch
+    out = {}
+    for storage, kind in (('symmetric', 3), ('full', 2)):
+        sol, _ = setup_solution(n, dev, chi_storage=storage)
+        assert sol.chi_kind == kind
+        sol.solve(callback=lambda s_: s_.iteration < 20)
+        wall, ev = time_iterations(sol, steps, warmup)
+        mm_b = must_move_bytes(3, 3, 8, kind, 1.0 / n)
+        out[storage] = {'chi_kind': kind, 'chi_bytes_per_voxel': {3: 48, 2: 72}[kind], 'ms_per_iteration': round(wall * 1e3, 4),
+                        'voxel_iter_per_s': round(n ** 3 / wall, 1), 'must_move_bytes_per_voxel_iter': round(mm_b, 1),
+                        'GBps_must_move': round(mm_b * n ** 3 / ev / 1e9, 1),
+                        'frac_must_move': round(mm_b * n ** 3 / ev / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f'chi storage {storage}: {wall * 1e3:.3f} ms per iteration')
+        del sol
+        torch.cuda.empty_cache()
+    return out
 
 
 # ---------------------------------------------------------------------------------------------------------------
@@ -580,6 +660,7 @@ def main():
     ap.add_argument('--no-batched', action='store_true', help='skip the batched-solve comparison')
     ap.add_argument('--no-tune', action='store_true', help='keep the library default tile shapes (no autotuning)')
     ap.add_argument('--no-accuracy', action='store_true', help='skip the complex128 reference solve')
+    ap.add_argument('--no-storage', action='store_true', help='skip the timing of the full and symmetric chi storage')
     ap.add_argument('--batched-only', action='store_true', help='only the batched-solve comparison (own JSON line)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the adjoint-gradient measurement')
     ap.add_argument('--no-magnetic', action='store_true', help='skip the magnetic-medium measurement')
@@ -637,47 +718,41 @@ def main():
         torch.cuda.synchronize()
         print(f'bench: copy probe of {args.copy_probe_mb} MiB done', file=sys.stderr, flush=True)
         del src, dst
-    for _ in range(args.warmup):
-        sol.step()
-    stream = torch.cuda.current_stream(dev)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        sol.step()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
-    ms_per_step = wall * 1e3 / args.steps
-    value = V * args.steps / wall
+    wall_per_iter, gpu_per_iter = time_iterations(sol, args.steps, args.warmup)
+    ms_per_step = wall_per_iter * 1e3
+    value = V / wall_per_iter
 
-    b_alg = 480.0  # bytes per voxel-iteration: 4*d*C*e + 2*B_chi + B_s + C*e with d=3, C=3, e=8, B_chi=72
-    gpu_ms_per_iter = gpu_ms / args.steps
-    achieved = b_alg * V / (gpu_ms_per_iter * 1e-3) / 1e9
+    chi_kind = sol.chi_kind
+    chi_bytes = {1: 8, 2: 72, 3: 48, 4: 28}[chi_kind]
+    # bytes the fused schedule must move (the source is the plane z = 20: 1/n of the voxels)
+    b_must = must_move_bytes(3, 3, 8, chi_kind, 1.0 / n)
+    b_alg = algorithmic_bytes(3, 3, 8, 9)  # the reference's sequence with its 3x3 chi: 480 B
+    achieved = b_must * V / gpu_per_iter / 1e9
     traffic = measured_traffic(n)
     roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': round(achieved / HBM_PEAK_GBS, 4),
                 'traffic': None if traffic is None else round(traffic * V),
                 'kernel': 'mmx_born_iterate: one Born iteration = 4 pencil-pass kernels + FP64 norm reduction, '
-                          'timed with HIP events on its launch stream',
-                'bytes_per_voxel_iter_alg': b_alg, 'gpu_ms_per_iter_events': round(gpu_ms_per_iter, 4)}
+                          'timed with HIP events on its launch stream; achieved = bytes the fused schedule must '
+                          'move per iteration / event time',
+                'bytes_per_voxel_iter': round(b_must, 2), 'gpu_ms_per_iter_events': round(gpu_per_iter * 1e3, 4),
+                'alg_bytes_per_voxel_iter': b_alg,
+                'alg_equivalent_GBps': round(b_alg * V / gpu_per_iter / 1e9, 1),
+                'alg_equivalent_frac': round(b_alg * V / gpu_per_iter / 1e9 / HBM_PEAK_GBS, 4)}
     if traffic is not None:
         roofline['traffic_bytes_per_voxel_iter'] = traffic
-        roofline['actual_GBps'] = round(traffic * V / (gpu_ms_per_iter * 1e-3) / 1e9, 1)
-        # the fused schedule moves fewer bytes than the algorithmic count, so frac (algorithmic) can exceed 1;
-        # frac_actual is the measured traffic's share of the peak
-        roofline['frac_actual'] = round(roofline['actual_GBps'] / HBM_PEAK_GBS, 4)
+        roofline['traffic_GBps'] = round(traffic * V / gpu_per_iter / 1e9, 1)
+        roofline['traffic_frac'] = round(roofline['traffic_GBps'] / HBM_PEAK_GBS, 4)
 
     out = {
         'metric': METRIC, 'value': round(value, 1), 'unit': 'voxel-iter/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': round(ms_per_step, 4), 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'c64', 'data': 'synthetic',
         'config': {'workload': f'{n}^3 anisotropic vectorial Born iteration (Voronoi grains, 2000 seeds), complex64',
-                   'grid': [n, n, n], 'components': 3, 'chi': '3x3 per voxel', 'pitch_nm': 250,
-                   'wavelength_nm': 1000},
+                   'grid': [n, n, n], 'components': 3, 'chi_kind': chi_kind,
+                   'chi': {1: 'isotropic', 2: 'full 3x3', 3: 'symmetric 3x3 (6 entries)',
+                           4: 'uniaxial u I + w n n^T'}[chi_kind] + f', {chi_bytes} B per voxel',
+                   'must_move_bytes_per_voxel_iter': round(b_must, 2), 'pitch_nm': 250, 'wavelength_nm': 1000},
         'hbm_fraction': roofline['frac'],
         'roofline': roofline,
         'time_to_residual_s': round(ttr, 4), 'iterations_to_residual': iters_to_res, 'residual': residue,
@@ -685,16 +760,25 @@ def main():
     }
     if tuned is not None:
         out['autotune'] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned.items()}
-    if not args.no_accuracy:
-        out['accuracy'] = accuracy_benchmark(n, dev, sol.field_array.clone(), residue, log)
-        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
-        print('bench: CPU baseline (one complex128 oracle iteration)', file=sys.stderr, flush=True)
+        log('CPU baseline (complex128 oracle iterations)')
         cb = cpu_baseline(sol, n)
         out['cpu_baseline'] = cb
         out['cpu_time_to_residual_s_extrapolated'] = round(cb['seconds_per_iteration'] * iters_to_res, 1)
-    if not args.no_batched:
+    if not args.no_accuracy:
+        E_plain = sol.field_array.clone()
         del sol
+        torch.cuda.empty_cache()
+        out['accuracy'] = accuracy_benchmark(n, dev, E_plain, residue, log)
+        out['time_to_accuracy_1e-4_s'] = out['accuracy']['refined_c64']['time_s']
+        out['accuracy_at_time_to_accuracy'] = out['accuracy']['refined_c64']['rel_l2_vs_c128']
+        del E_plain
+        torch.cuda.empty_cache()
+    else:
+        del sol
+    if not args.no_storage:
+        out['chi_storage'] = storage_benchmark(n, dev, log=log)
+    if not args.no_batched:
         torch.cuda.empty_cache()
         out['batched'] = batched_benchmark(log=log)
     if not args.no_adjoint:

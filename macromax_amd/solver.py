@@ -56,7 +56,15 @@ class Solution:
 
     def __init__(self, grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_wavelength=None,
                  current_density=None, source_distribution=None, epsilon=None, xi=0.0, zeta=0.0, mu=1.0,
-                 refractive_index=None, bound: Bound | None = None, initial_field=0.0, dtype=None):
+                 refractive_index=None, bound: Bound | None = None, initial_field=0.0, dtype=None,
+                 chi_storage: str = 'auto'):
+        """Arguments as MacroMax's ``Solution`` (``solver.py:93-268``), plus ``chi_storage`` (no reference
+        counterpart; how the kernels hold an anisotropic chi): 'auto' picks the most compact exact form, 'uniaxial'
+        (u I + w n n^T, 28 bytes per voxel in complex64), 'symmetric' (6 entries, 48 bytes) or 'full' (9 entries,
+        72 bytes); a form that does not hold the medium raises ValueError."""
+        if chi_storage not in ('auto', 'uniaxial', 'symmetric', 'full'):
+            raise ValueError(f"chi_storage must be 'auto', 'uniaxial', 'symmetric' or 'full', not {chi_storage!r}")
+        self.__chi_storage = chi_storage
         self.__iteration = 0
         if not isinstance(grid, Grid):
             if np.isscalar(grid[0]):
@@ -232,12 +240,18 @@ class Solution:
         # reciprocal media (eps^T = eps to 8 rounding units of the element type) are stored symmetrically
         symmetric = False
         uniaxial = None
-        if chi.shape[0] == 3:
+        storage = self.__chi_storage
+        if chi.shape[0] == 3 and storage != 'full':
             asym = torch.amax(torch.abs(eps64 - eps64.transpose(0, 1)))
             symmetric = bool(asym <= 8 * torch.finfo(BE.hardware_dtype).eps * torch.amax(torch.abs(eps64)))
             if symmetric:
                 chi = 0.5 * (chi + chi.transpose(0, 1))
-                uniaxial = _uniaxial_planes(chi, BE.hardware_dtype)
+                if storage != 'symmetric':
+                    uniaxial = _uniaxial_planes(chi, BE.hardware_dtype)
+            if storage == 'symmetric' and not symmetric:
+                raise ValueError("chi_storage='symmetric' needs a reciprocal permittivity (eps^T = eps)")
+            if storage == 'uniaxial' and uniaxial is None:
+                raise ValueError("chi_storage='uniaxial' needs eps = a I + b n n^T with a real axis n per voxel")
         if chi.shape[0] == 1:
             self.__chi_kind = 1
             self.__chi = chi.to(BE.hardware_dtype).contiguous()

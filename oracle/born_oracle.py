@@ -234,14 +234,23 @@ def longitudinal_projection_ft(F, k):
     return out
 
 
-def green_op(X, k, alpha, vectorial):
+def green_op(X, k, alpha, vectorial, workers=None):
     """G X = ift(g_op(ft(X))) with g = -i alpha_i / (k^2 - alpha)  (``solver.py:564-584``).
 
     Vectorial: F <- g (F - Pi_L F) + (i alpha_i / alpha) Pi_L F (``solver.py:570-577``).
-    FFT over the spatial axes 2.. (``backend/numpy.py:90-94``), unnormalised forward, 1/N inverse.
+    FFT over the spatial axes 2.. (``backend/numpy.py:90-94``), unnormalised forward, 1/N inverse: numpy.fft as the
+    reference's NumPy backend without pyFFTW; ``workers`` selects scipy.fft with that many threads instead (a tuned
+    CPU variant, as pyFFTW's threads would be, ``backend/numpy.py:46-89``).
     """
     axes = tuple(range(2, X.ndim))
-    F = np.fft.fftn(X, axes=axes)
+    if workers is not None:
+        import scipy.fft
+        fftn = lambda a: scipy.fft.fftn(a, axes=axes, workers=workers)
+        ifftn = lambda a: scipy.fft.ifftn(a, axes=axes, workers=workers)
+    else:
+        fftn = lambda a: np.fft.fftn(a, axes=axes)
+        ifftn = lambda a: np.fft.ifftn(a, axes=axes)
+    F = fftn(X)
     shape = X.shape[2:]
     k2 = sum(_kb(k[a], a, len(shape)) ** 2 for a in range(len(shape)))
     g = -1.0j * alpha.imag / (k2 - alpha)
@@ -250,7 +259,7 @@ def green_op(X, k, alpha, vectorial):
         F = g * (F - PL) + PL * (1.0j * alpha.imag / alpha)
     else:
         F = g * F
-    return np.fft.ifftn(F, axes=axes)
+    return ifftn(F)
 
 
 def chi_mul(chi, X):
@@ -267,7 +276,8 @@ class BornOracle:
     ``step()`` performs one cycle and returns True if the update was accepted.
     """
 
-    def __init__(self, chi, s, alpha, beta, k, vectorial, E0=None):
+    def __init__(self, chi, s, alpha, beta, k, vectorial, E0=None, workers=None):
+        self.workers = workers
         self.chi = np.asarray(chi, dtype=np.complex128)
         self.s = np.asarray(s, dtype=np.complex128)
         self.alpha = complex(alpha)
@@ -295,7 +305,7 @@ class BornOracle:
     def update(self):
         """d = chi [G (chi E + s) - E]  (``solver.py:1070-1075``)."""
         d = chi_mul(self.chi, self.E) + self.s
-        d = green_op(d, self.k, self.alpha, self.vectorial)
+        d = green_op(d, self.k, self.alpha, self.vectorial, self.workers)
         d = d - self.E
         return chi_mul(self.chi, d)
 

@@ -319,3 +319,43 @@ def test_uniaxial_storage(dtype):
     eps_b[0, 0] += 0.2  # biaxial
     sol_b = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps_b, current_density=J, bound=bound, dtype=dtype)
     assert sol_b.chi_kind == 3
+
+
+def test_chi_storage_option():
+    """chi_storage forces the full (kind 2) or symmetric (kind 3) storage of a uniaxial medium: the same operator, so
+    the same iterates (complex64, 1e-5); a storage that cannot hold the medium raises ValueError."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print('chi storage option', flush=True)
+    rng = np.random.default_rng(17)
+    grid = mm.Grid([16, 20, 24], 0.1e-6)
+    a = rng.standard_normal((3, *grid.shape))
+    a /= np.linalg.norm(a, axis=0)
+    eps = 1.5 ** 2 * np.eye(3).reshape(3, 3, 1, 1, 1) + (1.65 ** 2 - 1.5 ** 2) * a[:, None] * a[None, :]
+    J = np.zeros((3, *grid.shape), dtype=np.complex64)
+    J[0, :, :, 3] = 1
+    J[1, :, :, 3] = 0.5j
+    bound = LinearBound(grid, thickness=0.4e-6)
+    fields = {}
+    for storage, kind in (('auto', 4), ('uniaxial', 4), ('symmetric', 3), ('full', 2)):
+        sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, current_density=J, bound=bound,
+                          dtype=np.complex64, chi_storage=storage)
+        assert sol.chi_kind == kind, storage
+        sol.solve(lambda s_: s_.iteration < 8)
+        fields[storage] = sol.E
+    for storage in ('uniaxial', 'symmetric', 'full'):
+        err = np.linalg.norm(fields[storage] - fields['auto']) / np.linalg.norm(fields['auto'])
+        assert err < 1e-5, (storage, err)
+    eps_b = eps.copy()
+    eps_b[0, 0] += 0.2  # biaxial
+    with pytest.raises(ValueError):
+        mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps_b, current_density=J, bound=bound,
+                    dtype=np.complex64, chi_storage='uniaxial')
+    eps_n = eps_b.astype(np.complex128)
+    eps_n[0, 1] += 0.05j  # non-reciprocal
+    with pytest.raises(ValueError):
+        mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps_n, current_density=J, bound=bound,
+                    dtype=np.complex64, chi_storage='symmetric')
+    with pytest.raises(ValueError):
+        mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, current_density=J, bound=bound,
+                    dtype=np.complex64, chi_storage='packed')
