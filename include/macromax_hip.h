@@ -21,7 +21,7 @@ extern "C" {
 
 #define MMX_OK 0
 #define MMX_EINVAL -1    /* bad argument (shape, size, kind) */
-#define MMX_EUNSUP -2    /* unsupported FFT length (prime factor > 7, more than 4096 points, or too large for LDS) */
+#define MMX_EUNSUP -2    /* unsupported FFT length (a prime factor above 2048, or more than 4096^2 points) */
 #define MMX_EHIP -3      /* HIP runtime error */
 #define MMX_ENOMEM -4
 
@@ -31,12 +31,20 @@ typedef struct mmx_plan_s* mmx_plan_t;
 int mmx_version(void);
 /* Text of the last error on this thread (empty if none). */
 const char* mmx_last_error(void);
+/* 1 if complex64 axes of n points have register-resident FFT kernels (MMX_PARAM_REG_FFT chooses between them and the
+ * LDS-staged kernels), else 0. No reference counterpart (its FFTs are library calls); for tuners. */
+int mmx_register_fft_length(int64_t n);
 
 /*
  * Plan for a grid of `ndim` (1..3) axes with `shape[ndim]` samples and k0-scaled sample pitch `step[ndim]`.
  * Holds twiddle tables and the k-vectors k_a = 2 pi fftfreq(n_a, step_a) (MacroMax utils/ft/grid.py:426-439)
  * on the current device. Replaces the state that MacroMax BackEnd.__init__ builds
- * (backend/__init__.py:43-67; pyFFTW plans in backend/numpy.py:46-89).
+ * (backend/__init__.py:43-67; pyFFTW plans in backend/numpy.py:46-89). Axis lengths: up to 4096 points with prime
+ * factors up to 7 in one pass; a length with a larger prime factor (up to 2048 points) by Bluestein's algorithm in one
+ * pass; longer axes as a four-step product n1 x n2 of such lengths (two passes and a transposition, with a scratch
+ * buffer of one field batch allocated on first use). Grids that need Bluestein or four-step axes, or whose fused tiles
+ * do not fit in LDS (e.g. three complex128 components of 4096 points), run the Born iteration unfused: voxel-wise
+ * kernels around whole-grid FFTs, with the same results.
  */
 int mmx_plan_create(mmx_plan_t* plan, int ndim, const int64_t* shape, const double* step, int dtype);
 int mmx_plan_destroy(mmx_plan_t plan);

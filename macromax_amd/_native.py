@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libmacromax
 MMX_C64 = 0
 MMX_C128 = 1
 
-EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
+EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_register_fft_length', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
             'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_step_noalloc',
             'mmx_born_prologue',
             'mmx_born_iterate',
@@ -50,6 +50,8 @@ def _declare(L):
     L.mmx_version.argtypes = []
     L.mmx_last_error.restype = ctypes.c_char_p
     L.mmx_last_error.argtypes = []
+    L.mmx_register_fft_length.restype = c_int
+    L.mmx_register_fft_length.argtypes = [c_i64]
     L.mmx_plan_create.restype = c_int
     L.mmx_plan_create.argtypes = [ctypes.POINTER(c_p), c_int, ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), c_int]
     L.mmx_plan_destroy.restype = c_int
@@ -107,6 +109,11 @@ def wall_clock_rate_khz() -> int:
     v = ctypes.c_int64()
     check(lib().mmx_wall_clock_rate_khz(ctypes.byref(v)), 'mmx_wall_clock_rate_khz')
     return int(v.value)
+
+
+def register_fft_length(n: int) -> bool:
+    """Whether complex64 axes of n points have register-resident FFT kernels (mmx_register_fft_length)."""
+    return bool(lib().mmx_register_fft_length(int(n)))
 
 
 def check(rc: int, what: str = ''):

@@ -77,11 +77,12 @@ def apply(plan, params: dict):
 
 def _candidates(plan):
     out = {}
-    pow2 = [n for n in plan.shape if n > 1 and (n & (n - 1)) == 0 and 64 <= n <= 512]
+    from . import _native
+    reg = [n for n in plan.shape if n > 1 and _native.register_fft_length(n)]
     for name, values in PARAMETERS.items():
         if name == 'g_axis' and sum(1 for n in plan.shape if n > 1) < 3:
             continue
-        if name == 'reg_fft' and (not pow2 or plan.dtype != torch.complex64):
+        if name == 'reg_fft' and (not reg or plan.dtype != torch.complex64):
             continue
         if name == 'p_strided' and sum(1 for n in plan.shape if n > 1) < 3:
             continue

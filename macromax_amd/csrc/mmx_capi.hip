@@ -1,5 +1,6 @@
 // C ABI of libmacromax_hip.so: plans, pass scheduling and launches (declarations in include/macromax_hip.h).
 #include <atomic>
+#include <complex>
 #include <mutex>
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -10,6 +11,7 @@
 
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
+#include "mmx_pointwise.hpp"
 #include "mmx_regkernels.hpp"
 #include "mmx_setup.hpp"
 
@@ -45,12 +47,20 @@ constexpr int kElemsDBig = 8;
 
 struct AxisPlan {
     int orig;        // original grid axis (0..2)
-    int N;
+    int N;           // points along the axis
+    int NT;          // length of the FFTs that transform it in one pass: N, or the Bluestein length (> 2N - 2)
     int nrad;
-    int rad[kMaxRadices];
+    int rad[kMaxRadices];  // radices of NT
     int64_t inner;   // stride of the axis
     int64_t nouter;  // blocks before the axis (per field)
-    void* tw;        // device twiddles (forward), N complex
+    void* tw;        // device twiddles (forward), NT complex
+    int blue;        // 1: Bluestein's algorithm (N has a prime factor above 7)
+    void* chirp;     // N complex exp(-i pi n^2 / N)
+    void* bhat_f;    // NT complex: FFT of the wrapped conjugate chirp / NT (forward transforms)
+    void* bhat_i;    // the same for the inverse transform (the chirp itself)
+    int split;       // 1: four-step: the axis is N1 x N2 sub-axes (n = n1 N2 + n2), each transformed in its own pass
+    int N1, N2;
+    void* tw4;       // N complex exp(-2 pi i t / N), the four-step twiddles w^(k1 n2)
 };
 
 }  // namespace
@@ -62,6 +72,10 @@ struct mmx_plan_s {
     int64_t V;
     int naxes;
     AxisPlan ax[3];  // non-trivial axes, outermost first
+    AxisPlan sub[3][2];  // sub-axes of split axes: [0] along n1 (stride N2 inner), [1] along n2 (stride inner)
+    void* fft_scratch;   // one field batch: the four-step transposition's target
+    size_t fft_scratch_bytes;
+    int fused_ok[2];     // per component count (1, 3): -1 unknown, 1 the fused pass schedule fits, 0 unfused
     void* kvec;      // device k-vectors, n0 + n1 + n2 reals
     double step3[3]; // grid step per axis (1 for padded axes)
     int transposed;  // 1: k-vectors of the transposed operator (Nyquist entries negated)
@@ -95,6 +109,92 @@ bool factorize(int n, int* rad, int& nrad) {
         }
     }
     return m == 1;
+}
+
+bool smooth7(int64_t n) {
+    if (n < 1) return false;
+    for (int r : {2, 3, 5, 7})
+        while (n % r == 0) n /= r;
+    return n == 1;
+}
+
+int64_t next_smooth7(int64_t n) {
+    while (!smooth7(n)) ++n;
+    return n;
+}
+
+// One pass can transform an axis of n points directly (Stockham, n 7-smooth) or by Bluestein's algorithm (a 7-smooth
+// transform length >= 2n - 1), both at most kMaxN points in LDS.
+bool direct_ok(int64_t n) { return n <= kMaxN && smooth7(n); }
+bool blue_ok(int64_t n) { return n >= 2 && next_smooth7(2 * n - 1) <= kMaxN; }
+
+// Four-step split n = n1 n2 with both sub-lengths transformable in one pass: both 7-smooth if possible, then the
+// most balanced. Returns false when none exists (a prime factor above kMaxN / 2, or n beyond kMaxN^2).
+bool choose_split(int64_t n, int& n1, int& n2) {
+    int best_score = 1 << 30;
+    for (int64_t d = 2; d * d <= n; ++d) {
+        if (n % d) continue;
+        const int64_t e = n / d;
+        if (e > kMaxN) continue;
+        const bool ok_d = direct_ok(d) || blue_ok(d), ok_e = direct_ok(e) || blue_ok(e);
+        if (!ok_d || !ok_e) continue;
+        const int score = (direct_ok(d) ? 0 : 1 << 20) + (direct_ok(e) ? 0 : 1 << 20) + (int)(e - d);
+        if (score < best_score) {
+            best_score = score;
+            n1 = (int)d;
+            n2 = (int)e;
+        }
+    }
+    return best_score < (1 << 30);
+}
+
+template <typename T>
+int upload_complex(const std::vector<std::complex<double>>& h, void** dev) {
+    std::vector<cx<T>> tmp(h.size());
+    for (size_t i = 0; i < h.size(); ++i) tmp[i] = cx<T>{T(h[i].real()), T(h[i].imag())};
+    MMX_HIP_CHECK(hipMalloc(dev, tmp.size() * sizeof(cx<T>)));
+    MMX_HIP_CHECK(hipMemcpy(*dev, tmp.data(), tmp.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
+    return MMX_OK;
+}
+
+// Bluestein tables of an axis of N points with transform length NT (double precision on the host): the chirp
+// c_n = exp(-i pi n^2 / N) (n^2 reduced mod 2N first), and bhat = DFT_NT(b) / NT for the convolution kernel
+// b_m = conj(c_m) at m and NT - m (forward transforms) or c_m (inverse transforms).
+template <typename T>
+int make_bluestein(AxisPlan& ax) {
+    const int N = ax.N, NT = ax.NT;
+    std::vector<std::complex<double>> c(N), bf(NT, 0.0), bi(NT, 0.0), hf(NT), hi(NT), w(NT);
+    for (int n = 0; n < N; ++n) {
+        const int64_t q = ((int64_t)n * n) % (2 * (int64_t)N);
+        const double ang = -M_PI * double(q) / double(N);
+        c[n] = std::complex<double>(cos(ang), sin(ang));
+    }
+    for (int m = 0; m < N; ++m) {
+        bf[m] = std::conj(c[m]);
+        bi[m] = c[m];
+        if (m > 0) {
+            bf[NT - m] = std::conj(c[m]);
+            bi[NT - m] = c[m];
+        }
+    }
+    for (int t = 0; t < NT; ++t) {
+        const double ang = -2.0 * M_PI * double(t) / double(NT);
+        w[t] = std::complex<double>(cos(ang), sin(ang));
+    }
+    for (int k = 0; k < NT; ++k) {
+        std::complex<double> sf = 0.0, si = 0.0;
+        for (int m = 0; m < NT; ++m) {
+            const std::complex<double> e = w[(int64_t)m * k % NT];
+            sf += bf[m] * e;
+            si += bi[m] * e;
+        }
+        hf[k] = sf / double(NT);
+        hi[k] = si / double(NT);
+    }
+    int rc = upload_complex<T>(c, &ax.chirp);
+    if (!rc) rc = upload_complex<T>(hf, &ax.bhat_f);
+    if (!rc) rc = upload_complex<T>(hi, &ax.bhat_i);
+    return rc;
 }
 
 template <typename T>
@@ -144,6 +244,28 @@ int make_twiddles(int N, void** dev) {
     return MMX_OK;
 }
 
+// One pass's plan for an axis of n points: the Stockham radices and twiddles of the transform length, and the
+// Bluestein tables when n has a prime factor above 7.
+int init_axis(mmx_plan_s* p, AxisPlan& ax, int64_t n, int64_t inner, int64_t nouter) {
+    ax.N = (int)n;
+    ax.inner = inner;
+    ax.nouter = nouter;
+    ax.blue = 0;
+    ax.split = 0;
+    if (direct_ok(n)) {
+        ax.NT = (int)n;
+    } else if (blue_ok(n)) {
+        ax.NT = (int)next_smooth7(2 * n - 1);
+        ax.blue = 1;
+    } else {
+        return fail(MMX_EUNSUP, "internal: axis of " + std::to_string(n) + " points in one pass");
+    }
+    if (!factorize(ax.NT, ax.rad, ax.nrad)) return fail(MMX_EUNSUP, "internal: transform length radices");
+    int rc = p->dtype == MMX_C64 ? make_twiddles<float>(ax.NT, &ax.tw) : make_twiddles<double>(ax.NT, &ax.tw);
+    if (!rc && ax.blue) rc = p->dtype == MMX_C64 ? make_bluestein<float>(ax) : make_bluestein<double>(ax);
+    return rc;
+}
+
 int ensure_scratch(mmx_plan_s* p, size_t bytes) {
     if (p->scratch_bytes >= bytes) return MMX_OK;
     if (p->scratch) MMX_HIP_CHECK(hipFree(p->scratch));
@@ -179,12 +301,12 @@ int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_
                 int p_request = 0, size_t lds_budget = kLdsBudget, bool want_big = false) {
     int E = sizeof(T) == 4 ? kElemsF : kElemsD;
     t.big = 0;
-    if (want_big || (int64_t)(fused ? ncomp : 1) * ax.N > (int64_t)1024 * E) {
+    if (want_big || (int64_t)(fused ? ncomp : 1) * ax.NT > (int64_t)1024 * E) {
         E = sizeof(T) == 4 ? kElemsFBig : kElemsDBig;
         t.big = 1;
     }
     const size_t el = sizeof(cx<T>);
-    const int N = ax.N;
+    const int N = ax.NT;  // points per pencil in LDS
     const bool rows = ax.inner == 1;
     if (!fused) ncomp = 1;
     t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
@@ -231,12 +353,15 @@ int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_
 template <typename T>
 void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.N = ax.N;
+    a.NT = ax.NT;
+    a.blue = ax.blue;
+    a.chirp = reinterpret_cast<const cx<T>*>(ax.chirp);
     a.rp.nrad = ax.nrad;
     int prod = 1;
     for (int r = 0; r < kMaxRadices; ++r) {
         const int R = r < ax.nrad ? ax.rad[r] : 1;
         a.rp.rad[r] = R;
-        a.rp.per[r] = make_fdiv(ax.N / R);
+        a.rp.per[r] = make_fdiv(ax.NT / R);
         a.rp.p[r] = make_fdiv(prod);
         prod *= R;
     }
@@ -282,6 +407,7 @@ constexpr int reg_elems(int N, int NC, int KIND) {
 // Lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the strided extent
 // N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
 bool reg_supported(const AxisPlan& ax) {
+    if (ax.blue || ax.split || ax.NT != ax.N) return false;
     const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
                      ax.N == 384 || ax.N == 80 || ax.N == 160 || ax.N == 320 || ax.N == 640 || ax.N == 768 ||
                      ax.N == 1024 || ax.N == 2048;
@@ -475,8 +601,12 @@ int launch_pass_reg(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<float> a, 
 template <typename T>
 int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t nbatch, bool fused, int kind,
                 hipStream_t stream) {
+    if (ax.split) return fail(MMX_EINVAL, "internal: a split axis has no single pass");
+    if ((ax.blue || a.tw4) && kind != kFwd && kind != kInv)
+        return fail(MMX_EINVAL, "internal: Bluestein and four-step transforms run in plain passes only");
+    a.bhat = reinterpret_cast<const cx<T>*>(kind == kInv ? ax.bhat_i : ax.bhat_f);
     if constexpr (sizeof(T) == 4) {
-        if (p->reg_fft && reg_supported(ax)) return launch_pass_reg(p, ax, a, nbatch, fused, kind, stream);
+        if (p->reg_fft && reg_supported(ax) && !a.tw4) return launch_pass_reg(p, ax, a, nbatch, fused, kind, stream);
     }
     TileCfg t;
     const bool rows = ax.inner == 1;
@@ -498,7 +628,7 @@ int launch_pass(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T> a, int64_t 
     a.fncomp = make_fdiv(a.ncomp > 0 ? a.ncomp : 1);
     a.fntile = make_fdiv64(t.ntile_inner);
     a.ftpp = make_fdiv64(t.tiles_per_problem);
-    if ((int64_t)a.ncomp * t.P * (ax.N + t.pad) >= (1 << 22)) return fail(MMX_EUNSUP, "internal: tile index range");
+    if ((int64_t)a.ncomp * t.P * (ax.NT + t.pad) >= (1 << 22)) return fail(MMX_EUNSUP, "internal: tile index range");
     if (t.nblocks <= 0) return MMX_OK;
     if (t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "grid too large");
     switch (kind) {
@@ -525,26 +655,91 @@ PassArgs<T> blank_args() {
     return a;
 }
 
+int ensure_fft_scratch(mmx_plan_s* p, size_t bytes) {
+    if (p->fft_scratch_bytes >= bytes) return MMX_OK;
+    if (p->fft_scratch) MMX_HIP_CHECK(hipFree(p->fft_scratch));
+    p->fft_scratch = nullptr;
+    p->fft_scratch_bytes = 0;
+    MMX_HIP_CHECK(hipMalloc(&p->fft_scratch, bytes));
+    p->fft_scratch_bytes = bytes;
+    ++p->scratch_gen;
+    return MMX_OK;
+}
+
+bool has_split(const mmx_plan_s* p) {
+    for (int q = 0; q < p->naxes; ++q)
+        if (p->ax[q].split) return true;
+    return false;
+}
+
+// FFT of `nfields` consecutive fields over every axis, innermost first. scale_inverse: the inverse transform's 1/V
+// goes on the last pass (the unfused Born iteration folds it into its Green's operator instead).
 template <typename T>
-int fft_impl(mmx_plan_s* p, const void* in, void* out, int64_t nbatch, int direction, hipStream_t stream) {
-    if (nbatch <= 0) return MMX_OK;
+int fft_passes(mmx_plan_s* p, const void* in, void* out, int64_t nfields, int direction, bool scale_inverse,
+               hipStream_t stream) {
+    if (nfields <= 0) return MMX_OK;
     if (p->naxes == 0) {
-        if (in != out) MMX_HIP_CHECK(hipMemcpyAsync(out, in, nbatch * p->V * sizeof(cx<T>), hipMemcpyDeviceToDevice, stream));
+        if (in != out)
+            MMX_HIP_CHECK(hipMemcpyAsync(out, in, nfields * p->V * sizeof(cx<T>), hipMemcpyDeviceToDevice, stream));
         return MMX_OK;
     }
+    if (has_split(p)) {
+        const int rc = ensure_fft_scratch(p, (size_t)nfields * p->V * sizeof(cx<T>));
+        if (rc) return rc;
+    }
+    const int kind = direction < 0 ? kFwd : kInv;
     const void* src = in;
     for (int q = p->naxes - 1; q >= 0; --q) {
-        PassArgs<T> a = blank_args<T>();
-        fill_geometry<T>(p, p->ax[q], a);
-        a.src = reinterpret_cast<const cx<T>*>(src);
-        a.dst = reinterpret_cast<cx<T>*>(out);
-        a.dir = direction < 0 ? -1 : 1;
-        a.scale = (direction > 0 && q == 0) ? T(1.0 / double(p->V)) : T(1);
-        int rc = launch_pass<T>(p, p->ax[q], a, nbatch, false, direction < 0 ? kFwd : kInv, stream);
-        if (rc) return rc;
+        const AxisPlan& ax = p->ax[q];
+        const T scale = (direction > 0 && q == 0 && scale_inverse) ? T(1.0 / double(p->V)) : T(1);
+        if (!ax.split) {
+            PassArgs<T> a = blank_args<T>();
+            fill_geometry<T>(p, ax, a);
+            a.src = reinterpret_cast<const cx<T>*>(src);
+            a.dst = reinterpret_cast<cx<T>*>(out);
+            a.dir = direction < 0 ? -1 : 1;
+            a.scale = scale;
+            int rc = launch_pass<T>(p, ax, a, nfields, false, kind, stream);
+            if (rc) return rc;
+        } else {
+            // four-step: length-N1 FFTs along n1, then the twiddled length-N2 FFTs along n2 into the scratch buffer,
+            // then the transposition of (k1, k2) to the natural order k = k1 + N1 k2
+            cx<T>* scr = reinterpret_cast<cx<T>*>(p->fft_scratch);
+            PassArgs<T> a = blank_args<T>();
+            fill_geometry<T>(p, p->sub[q][0], a);
+            a.src = reinterpret_cast<const cx<T>*>(src);
+            a.dst = reinterpret_cast<cx<T>*>(out);
+            a.dir = direction < 0 ? -1 : 1;
+            int rc = launch_pass<T>(p, p->sub[q][0], a, nfields, false, kind, stream);
+            if (rc) return rc;
+            PassArgs<T> b = blank_args<T>();
+            fill_geometry<T>(p, p->sub[q][1], b);
+            b.src = reinterpret_cast<const cx<T>*>(out);
+            b.dst = scr;
+            b.dir = a.dir;
+            b.scale = scale;
+            b.tw4 = 1;
+            b.tw4tab = reinterpret_cast<const cx<T>*>(ax.tw4);
+            b.ftw4 = make_fdiv64(ax.N1);
+            rc = launch_pass<T>(p, p->sub[q][1], b, nfields, false, kind, stream);
+            if (rc) return rc;
+            const int64_t outer = nfields * ax.nouter;
+            int64_t nblk = ax.inner >= 64 ? (outer * ax.N * ax.inner + 255) / 256
+                                          : outer * ax.inner * ((ax.N1 + 31) / 32) * ((ax.N2 + 31) / 32);
+            if (nblk > 65536) nblk = 65536;
+            if (nblk < 1) nblk = 1;
+            hipLaunchKernelGGL((axis_transpose_kernel<T>), dim3((unsigned)nblk), dim3(256), 0, stream, scr,
+                               reinterpret_cast<cx<T>*>(out), outer, (int64_t)ax.N1, (int64_t)ax.N2, ax.inner);
+            MMX_HIP_CHECK(hipGetLastError());
+        }
         src = out;
     }
     return MMX_OK;
+}
+
+template <typename T>
+int fft_impl(mmx_plan_s* p, const void* in, void* out, int64_t nbatch, int direction, hipStream_t stream) {
+    return fft_passes<T>(p, in, out, nbatch, direction, true, stream);
 }
 
 // Tile configuration of the pass that writes the norm partials (same choice as launch_pass makes).
@@ -598,6 +793,87 @@ PassArgs<T> born_base(const mmx_plan_s* p, const void* E, void* Eout, const void
     return base;
 }
 
+// ---- the unfused iteration: voxel-wise kernels around whole-grid FFTs (axes that need Bluestein's algorithm or the
+// four-step split, or fused tiles that do not fit in LDS) ----
+
+// Whether the fused pass schedule serves `C` components on this plan (cached per plan; tuning parameters reset it).
+template <typename T>
+bool fused_schedule_fits(mmx_plan_s* p, int C) {
+    int& f = p->fused_ok[C == 3 ? 1 : 0];
+    if (f >= 0) return f == 1;
+    bool ok = true;
+    for (int q = 0; q < p->naxes; ++q)
+        if (p->ax[q].blue || p->ax[q].split) ok = false;
+    if (ok && p->naxes > 0) {
+        const std::string saved = g_last_error;
+        const int d = p->naxes;
+        TileCfg t;
+        const int kinds1[] = {kAll, kGreenEpiPro};
+        const int kinds3[] = {kProFwd, kInvEpi, kEpiPro};
+        const int* kinds = d == 1 ? kinds1 : kinds3;
+        const int nk = d == 1 ? 2 : 3;
+        for (int i = 0; i < nk && ok; ++i)
+            if (last_pass_tiles<T>(p, p->ax[d - 1], C, 1, kinds[i], t)) ok = false;
+        if (ok && d >= 2) {
+            int g = p->g_axis < 0 ? 0 : p->g_axis;
+            if (g > d - 2) g = d - 2;
+            const AxisPlan& ax = p->ax[g];
+            const int rc = (sizeof(T) == 4 && p->reg_fft && reg_supported(ax))
+                               ? reg_tile_config(ax, C, true, 1, p->p_green, kGreen, t)
+                               : tile_config<T>(ax, C, true, 1, p->lds_max, t, p->p_green, p->lds_budget,
+                                                (p->big_elems >> kGreen) & 1);
+            if (rc) ok = false;
+        }
+        g_last_error = saved;
+    }
+    f = ok ? 1 : 0;
+    return ok;
+}
+
+int pointwise_blocks(int64_t V) {
+    int64_t g = (V + 255) / 256;
+    if (g > 1024) g = 1024;
+    return (int)(g < 1 ? 1 : g);
+}
+
+// dst = chi_eff E + s_eff per voxel (E from Eout when src_from_out: the speculative next prologue).
+template <typename T>
+int unfused_prologue(mmx_plan_s* p, PassArgs<T> a, void* dst, int64_t nbatch, int src_from_out, hipStream_t stream) {
+    if (nbatch > 65535) return fail(MMX_EINVAL, "more than 65535 problems per call on a grid of this shape");
+    fill_geometry<T>(p, p->ax[0], a);
+    a.dst = reinterpret_cast<cx<T>*>(dst);
+    a.bstride = (int64_t)a.C * p->V;
+    a.comp_stride = p->V;
+    hipLaunchKernelGGL((born_pro_kernel<T>), dim3(pointwise_blocks(p->V), (unsigned)nbatch), dim3(256), 0, stream, a,
+                       src_from_out);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+// From W = chi_eff E + s_eff: W <- G W (forward FFT, Green's operator, inverse FFT), then E' = E + chi_eff (W - E)
+// with the norm partials (pointwise_blocks(V) per problem).
+template <typename T>
+int unfused_core(mmx_plan_s* p, PassArgs<T> a, void* W, int64_t nbatch, hipStream_t stream) {
+    if (nbatch > 65535) return fail(MMX_EINVAL, "more than 65535 problems per call on a grid of this shape");
+    const int64_t nf = nbatch * a.C;
+    int rc = fft_passes<T>(p, W, W, nf, -1, false, stream);
+    if (rc) return rc;
+    fill_geometry<T>(p, p->ax[0], a);
+    a.bstride = (int64_t)a.C * p->V;
+    a.comp_stride = p->V;
+    a.vectorial = (a.C == 3);
+    a.src = reinterpret_cast<const cx<T>*>(W);
+    a.dst = reinterpret_cast<cx<T>*>(W);
+    const dim3 grid(pointwise_blocks(p->V), (unsigned)nbatch);
+    hipLaunchKernelGGL((green_pointwise_kernel<T>), grid, dim3(256), 0, stream, a);
+    MMX_HIP_CHECK(hipGetLastError());
+    rc = fft_passes<T>(p, W, W, nf, +1, false, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL((born_epi_kernel<T>), grid, dim3(256), 0, stream, a);
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
 // W = forward FFT along the innermost axis of chi_eff E + s_eff (the first pass of an iteration).
 template <typename T>
 int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const void* chi, int C, int chi_kind,
@@ -611,6 +887,7 @@ int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const vo
                                  nullptr);
     a.state = state;
     a.only_stale = state != nullptr;
+    if (!fused_schedule_fits<T>(p, C)) return unfused_prologue<T>(p, a, W, nbatch, 0, stream);  // W = chi E + s
     fill_geometry<T>(p, inner, a);
     a.ncomp = C;
     a.bstride = (int64_t)C * p->V;
@@ -628,6 +905,18 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
     int rc = born_check<T>(p, C, chi_kind);
     if (rc) return rc;
     if (nbatch <= 0) return MMX_OK;
+    if (!fused_schedule_fits<T>(p, C)) {
+        const int64_t tiles = pointwise_blocks(p->V);
+        rc = ensure_scratch(p, (size_t)(2 * tiles * nbatch) * sizeof(double));
+        if (rc) return rc;
+        double* partials = reinterpret_cast<double*>(p->scratch);
+        PassArgs<T> a = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
+        a.state = state;
+        rc = unfused_core<T>(p, a, Wv, nbatch, stream);
+        if (!rc) rc = unfused_prologue<T>(p, a, Wv, nbatch, 1, stream);
+        if (!rc) rc = reduce_norms(partials, tiles, nbatch, norms, stream);
+        return rc;
+    }
     const int d = p->naxes;
     const AxisPlan& inner = p->ax[d - 1];
     const int last_kind = d == 1 ? kGreenEpiPro : kEpiPro;
@@ -703,9 +992,14 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
     if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
     const int d = p->naxes;
     const AxisPlan& inner = p->ax[d - 1];
+    const bool fused = fused_schedule_fits<T>(p, C);
+    if (!fused && fixed_partials != nullptr && has_split(p))
+        return fail(MMX_EINVAL, "this grid's four-step FFT needs a scratch buffer: use mmx_born_step");
     // partial sums: one pair per tile of the last pass
     TileCfg tl;
-    int rc = last_pass_tiles<T>(p, inner, C, nbatch, d == 1 ? kAll : kInvEpi, tl);
+    int rc = MMX_OK;
+    if (fused) rc = last_pass_tiles<T>(p, inner, C, nbatch, d == 1 ? kAll : kInvEpi, tl);
+    else tl.tiles_per_problem = pointwise_blocks(p->V);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
     const size_t partial_bytes = (size_t)(2 * tiles_last * nbatch) * sizeof(double);
@@ -724,6 +1018,12 @@ int born_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const voi
     }
     PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
     cx<T>* W = reinterpret_cast<cx<T>*>(Eout);
+    if (!fused) {  // Eout is the work buffer: x = chi E + s, G x, then E' = E + chi (G x - E) voxel by voxel
+        rc = unfused_prologue<T>(p, base, W, nbatch, 0, stream);
+        if (!rc) rc = unfused_core<T>(p, base, W, nbatch, stream);
+        if (!rc) rc = reduce_norms(partials, tiles_last, nbatch, norms, stream);
+        return rc;
+    }
 
     if (d == 1) {
         PassArgs<T> a = base;
@@ -884,6 +1184,14 @@ extern "C" {
 
 int mmx_version(void) { return 100; }
 
+int mmx_register_fft_length(int64_t n) {
+    if (n < 1 || n > kMaxN) return 0;
+    AxisPlan ax{};
+    ax.N = ax.NT = (int)n;
+    ax.inner = 1;
+    return reg_supported(ax) ? 1 : 0;
+}
+
 const char* mmx_last_error(void) { return g_last_error.c_str(); }
 
 int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const double* step, int dtype) {
@@ -914,25 +1222,40 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
     p->xcd_remap = (1 << kGreen) | (1 << kInvEpi) | (1 << kEpiPro) | (1 << kGreenEpiPro);
     p->reg_fft = 1;
     p->lds_budget = kLdsBudget;
+    p->fused_ok[0] = p->fused_ok[1] = -1;
     // non-trivial axes, outermost first
     p->naxes = 0;
     for (int a = 0; a < 3; ++a) {
         const int64_t n = p->shape3[a];
         if (n == 1) continue;
-        if (n > kMaxN) { mmx_plan_destroy(p); return fail(MMX_EUNSUP, "axis longer than 4096 samples"); }
-        AxisPlan& ax = p->ax[p->naxes++];
+        const int q = p->naxes++;
+        AxisPlan& ax = p->ax[q];
         ax.orig = a;
-        ax.N = (int)n;
-        if (!factorize(ax.N, ax.rad, ax.nrad)) {
-            mmx_plan_destroy(p);  // frees the twiddle tables of the axes before this one
-            return fail(MMX_EUNSUP, "FFT length " + std::to_string(n) + " has a prime factor above 7");
-        }
         int64_t inner = 1, outer = 1;
         for (int b = a + 1; b < 3; ++b) inner *= p->shape3[b];
         for (int b = 0; b < a; ++b) outer *= p->shape3[b];
-        ax.inner = inner;
-        ax.nouter = outer;
-        int rc = dtype == MMX_C64 ? make_twiddles<float>(ax.N, &ax.tw) : make_twiddles<double>(ax.N, &ax.tw);
+        int rc;
+        if (direct_ok(n) || blue_ok(n)) {
+            rc = init_axis(p, ax, n, inner, outer);
+        } else {
+            int n1 = 0, n2 = 0;
+            if (n > (int64_t)kMaxN * kMaxN || !choose_split(n, n1, n2)) {
+                mmx_plan_destroy(p);  // frees the tables of the axes before this one
+                return fail(MMX_EUNSUP, "FFT length " + std::to_string(n) +
+                                            " is not supported (a prime factor above 2048, or more than 4096^2 points)");
+            }
+            ax.N = (int)n;
+            ax.NT = (int)n;
+            ax.inner = inner;
+            ax.nouter = outer;
+            ax.split = 1;
+            ax.N1 = n1;
+            ax.N2 = n2;
+            p->sub[q][0].orig = p->sub[q][1].orig = a;
+            rc = init_axis(p, p->sub[q][0], n1, (int64_t)n2 * inner, outer);
+            if (!rc) rc = init_axis(p, p->sub[q][1], n2, inner, outer * n1);
+            if (!rc) rc = dtype == MMX_C64 ? make_twiddles<float>((int)n, &ax.tw4) : make_twiddles<double>((int)n, &ax.tw4);
+        }
         if (rc) { mmx_plan_destroy(p); return rc; }
     }
     for (int a = 0; a < 3; ++a) p->step3[a] = a < ndim ? step[a] : 1.0;
@@ -949,10 +1272,18 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
 
 int mmx_plan_destroy(mmx_plan_t p) {
     if (!p) return MMX_OK;
-    for (int a = 0; a < p->naxes; ++a)
-        if (p->ax[a].tw) (void)hipFree(p->ax[a].tw);
+    auto free_axis = [](AxisPlan& ax) {
+        for (void* d : {ax.tw, ax.chirp, ax.bhat_f, ax.bhat_i, ax.tw4})
+            if (d) (void)hipFree(d);
+    };
+    for (int a = 0; a < p->naxes; ++a) {
+        free_axis(p->ax[a]);
+        free_axis(p->sub[a][0]);
+        free_axis(p->sub[a][1]);
+    }
     if (p->kvec) (void)hipFree(p->kvec);
     if (p->scratch) (void)hipFree(p->scratch);
+    if (p->fft_scratch) (void)hipFree(p->fft_scratch);
     delete p;
     return MMX_OK;
 }
@@ -961,6 +1292,7 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
     g_last_error.clear();
     if (!p) return fail(MMX_EINVAL, "null plan");
     if (value < 0) return fail(MMX_EINVAL, "parameter values are >= 0");
+    p->fused_ok[0] = p->fused_ok[1] = -1;
     switch (param) {
         case MMX_PARAM_G_AXIS: p->g_axis = (int)value; break;
         case MMX_PARAM_P_ROWS: p->p_rows = (int)value; break;

@@ -49,6 +49,7 @@ struct FDiv {
 };
 
 __host__ inline FDiv make_fdiv(int d) { return FDiv{d, 1.0f / float(d)}; }
+__device__ __forceinline__ FDiv make_fdiv_dev(int d) { return FDiv{d, 1.0f / float(d)}; }
 
 __device__ __forceinline__ int fdiv(int x, const FDiv& f) {
     int q = __float2int_rz(__int2float_rn(x) * f.inv);

@@ -79,6 +79,14 @@ struct PassArgs {
     int s_restricted;
     int s_lo[3], s_hi[3];
     FDiv64 frpp;              // single-component passes: outer index o -> problem (o / (C * outer extent))
+    // transforms that are not one plain Stockham FFT of the pencil (plain passes only)
+    int NT;                   // transform length held in LDS per pencil: N, or the Bluestein length
+    int blue;                 // 1: Bluestein: chirp, FFT of length NT, product with bhat, inverse FFT, chirp
+    const cx<T>* chirp;       // N values exp(-i pi n^2 / N), conjugated in the kernel for the inverse transform
+    const cx<T>* bhat;        // NT values: FFT of the wrapped conjugate chirp of this pass's direction, / NT
+    int tw4;                  // 1: four-step sub-pass: point n of a pencil whose outer index is r is multiplied by
+    const cx<T>* tw4tab;      //    w^(DIR k1 n) before the FFT, k1 = r mod tw4_N1, w = tw4tab[1] = exp(-2 pi i / N4)
+    FDiv64 ftw4;              //    division by tw4_N1
     // Green's function
     int64_t n0, n1, n2;       // grid shape padded with trailing 1s
     const T* kvec;            // k_0 at [0, n0), k_1 at [k1ofs, +n1), k_2 at [k2ofs, +n2)
@@ -262,13 +270,81 @@ struct PassFlags {
     static constexpr bool STORE = !(KIND == kInvEpi || KIND == kAll);
 };
 
+// Position of point j of pencil p (one component) in a tile.
+__device__ __forceinline__ int tile_pos(const TileGeom& g, int p, int j) { return j * g.SJ + p * g.SM; }
+
+// The transform of a plain pass (one component per pencil) in the tile: the four-step twiddles (if any), then either
+// the Stockham FFT of length NT = N or Bluestein's algorithm (X_k = c_k sum_n (x_n c_n) conj(c_(k-n)) with the chirp
+// c_n = exp(-i pi n^2 / N), the circular convolution of length NT >= 2N - 1 as FFT, product with bhat, inverse FFT;
+// conjugated chirps for the inverse transform). Points N..NT-1 of each pencil are scratch.
+template <typename T, int DIR, int E>
+__device__ void plain_transform(cx<T>* tile, const cx<T>* tw, const TileGeom& g, const PassArgs<T>& a, int tid, int nth,
+                                int64_t r0, int64_t o, bool rows) {
+    const int N = a.N, P = a.P;
+    const int nv = P * N;
+    if (a.tw4) {
+        for (int idx = tid; idx < nv; idx += nth) {
+            int p, j;
+            if (rows) { p = fdiv(idx, a.fN); j = idx - p * N; }
+            else { j = fdiv(idx, a.fP); p = idx - j * P; }
+            const int64_t r = rows ? r0 + p : o;
+            const int64_t k1 = r - fdiv64(r, a.ftw4) * a.ftw4.d;
+            cx<T> w = a.tw4tab[k1 * j];
+            if (DIR > 0) w.i = -w.i;
+            const int q = tile_pos(g, p, j);
+            tile[q] = tile[q] * w;
+        }
+        __syncthreads();
+    }
+    if (!a.blue) {
+        tile_fft<T, DIR, E>(tile, tw, g, a.rp, tid, nth);
+        return;
+    }
+    const int NT = a.NT;
+    const FDiv fNT = make_fdiv_dev(NT);
+    for (int idx = tid; idx < P * NT; idx += nth) {
+        int p, j;
+        if (rows) { p = fdiv(idx, fNT); j = idx - p * NT; }
+        else { j = fdiv(idx, a.fP); p = idx - j * P; }
+        const int q = tile_pos(g, p, j);
+        if (j < N) {
+            cx<T> c = a.chirp[j];
+            if (DIR > 0) c.i = -c.i;
+            tile[q] = tile[q] * c;
+        } else {
+            tile[q] = cx<T>{T(0), T(0)};
+        }
+    }
+    __syncthreads();
+    tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
+    for (int idx = tid; idx < P * NT; idx += nth) {
+        int p, j;
+        if (rows) { p = fdiv(idx, fNT); j = idx - p * NT; }
+        else { j = fdiv(idx, a.fP); p = idx - j * P; }
+        const int q = tile_pos(g, p, j);
+        tile[q] = tile[q] * a.bhat[j];
+    }
+    __syncthreads();
+    tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
+    for (int idx = tid; idx < nv; idx += nth) {
+        int p, j;
+        if (rows) { p = fdiv(idx, a.fN); j = idx - p * N; }
+        else { j = fdiv(idx, a.fP); p = idx - j * P; }
+        cx<T> c = a.chirp[j];
+        if (DIR > 0) c.i = -c.i;
+        const int q = tile_pos(g, p, j);
+        tile[q] = tile[q] * c;
+    }
+    __syncthreads();
+}
+
 template <typename T, int E, int KIND>
 __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     using F = PassFlags<KIND>;
     extern __shared__ unsigned char smem_raw[];
     __shared__ double red[16];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
-    cx<T>* tile = tw + a.N;
+    cx<T>* tile = tw + a.NT;
     const int tid = threadIdx.x;
     const int nth = blockDim.x;
     const int N = a.N;
@@ -291,9 +367,9 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     }
 
     TileGeom g;
-    g.N = N;
+    g.N = a.NT;
     g.M = a.ncomp * a.P;
-    if (rows) { g.SJ = 1; g.SM = N + a.pad; g.m_fast = 0; }
+    if (rows) { g.SJ = 1; g.SM = a.NT + a.pad; g.m_fast = 0; }
     else { g.SJ = g.M; g.SM = 1; g.m_fast = 1; }
     g.fM.d = g.M;
     g.fM.inv = 1.0f / float(g.M);
@@ -317,7 +393,7 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         }
     }
 
-    for (int q = tid; q < N; q += nth) tw[q] = a.tw[q];
+    for (int q = tid; q < a.NT; q += nth) tw[q] = a.tw[q];
 
     // ---------------- load ----------------
     if constexpr (F::LOAD_PRO) {
@@ -377,7 +453,8 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
     __syncthreads();
 
     // ---------------- FFT (and Green) ----------------
-    if constexpr (F::FWD_FIRST) tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
+    if constexpr (KIND == kFwd) plain_transform<T, -1, E>(tile, tw, g, a, tid, nth, r0, o, rows);
+    else if constexpr (F::FWD_FIRST) tile_fft<T, -1, E>(tile, tw, g, a.rp, tid, nth);
     if constexpr (F::GREEN) {
         const int nvox = a.P * N;
         for (int idx = tid; idx < nvox; idx += nth) {
@@ -399,7 +476,8 @@ __global__ void __launch_bounds__(1024) pencil_pass_kernel(PassArgs<T> a) {
         }
         __syncthreads();
     }
-    if constexpr (F::INV) tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
+    if constexpr (KIND == kInv) plain_transform<T, +1, E>(tile, tw, g, a, tid, nth, r0, o, rows);
+    else if constexpr (F::INV) tile_fft<T, +1, E>(tile, tw, g, a.rp, tid, nth);
 
     // ---------------- store ----------------
     if constexpr (F::EPI) {

@@ -19,32 +19,61 @@ template <typename T> __device__ __forceinline__ cx<T>& at(cx<T>* base, uint32_t
     return *reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(base) + boff);
 }
 
+// Field traffic of the passes is streamed: every value is read once and written once per pass, and a field
+// (hundreds of MiB) does not fit the caches, so loads and stores carry the non-temporal hint (the "nt" cache policy).
+// Measured on MI355X with 7 read and 6 written planes of 128 MiB: 4.8 TB/s with default loads and stores, 5.5 TB/s
+// non-temporal; 3 + 3 planes: 4.9 -> 5.7 TB/s.
+// The fused Green pass is the exception: its strided tiles move 64-byte row segments, which neighbouring tiles on
+// one XCD complete to whole cache lines in L2; with the hint it took 227 instead of 173 us at 256^3.
+#ifndef MMX_NT
+#define MMX_NT 1
+#endif
+template <int KIND> constexpr bool nt_kind() { return MMX_NT && KIND != kGreen; }
+template <bool NT, typename T> __device__ __forceinline__ cx<T> ldg(const cx<T>* base, uint32_t boff) {
+    const cx<T>& r = at(base, boff);
+    if constexpr (NT) return cx<T>{__builtin_nontemporal_load(&r.r), __builtin_nontemporal_load(&r.i)};
+    else return r;
+}
+template <bool NT, typename T> __device__ __forceinline__ T ldg_real(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ void stg(cx<T>* base, uint32_t boff, cx<T> v) {
+    cx<T>& r = at(base, boff);
+    if constexpr (NT) {
+        __builtin_nontemporal_store(v.r, &r.r);
+        __builtin_nontemporal_store(v.i, &r.i);
+    } else {
+        r = v;
+    }
+}
+
 // chi of one voxel for a compile-time chi kind CK (1: scalar, 2: full 3x3 row-major, 3: packed symmetric,
 // 4: uniaxial u I + w n n^T).
 // chi_t is the chi of the tile (chi + tile offset tb, in complex elements), boff the byte offset of the voxel from it.
-template <typename T, int CK>
+template <bool NT, typename T, int CK>
 __device__ __forceinline__ void chi_load_ck(const cx<T>* chi_t, int64_t V, uint32_t boff, ChiVox<T>& q, int64_t tb) {
-    q.m0 = at(chi_t, boff);
+    q.m0 = ldg<NT>(chi_t, boff);
     if constexpr (CK == 4) {
-        q.m1 = at(chi_t + V, boff);
-        q.m2 = at(chi_t + 2 * V, boff);
+        q.m1 = ldg<NT>(chi_t + V, boff);
+        q.m2 = ldg<NT>(chi_t + 2 * V, boff);
         // n_z: the reals of the last plane, element tb + boff / sizeof(cx) of that real array
         const T* nz_t = reinterpret_cast<const T*>(chi_t - tb + 3 * V) + tb;
-        q.m3.r = *reinterpret_cast<const T*>(reinterpret_cast<const char*>(nz_t) + boff / 2);
+        q.m3.r = ldg_real<NT>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(nz_t) + boff / 2));
         q.m3.i = T(0);
         return;
     }
     if constexpr (CK >= 2) {
-        q.m1 = at(chi_t + V, boff);
-        q.m2 = at(chi_t + 2 * V, boff);
-        q.m3 = at(chi_t + 3 * V, boff);
-        q.m4 = at(chi_t + 4 * V, boff);
-        q.m5 = at(chi_t + 5 * V, boff);
+        q.m1 = ldg<NT>(chi_t + V, boff);
+        q.m2 = ldg<NT>(chi_t + 2 * V, boff);
+        q.m3 = ldg<NT>(chi_t + 3 * V, boff);
+        q.m4 = ldg<NT>(chi_t + 4 * V, boff);
+        q.m5 = ldg<NT>(chi_t + 5 * V, boff);
     }
     if constexpr (CK == 2) {
-        q.m6 = at(chi_t + 6 * V, boff);
-        q.m7 = at(chi_t + 7 * V, boff);
-        q.m8 = at(chi_t + 8 * V, boff);
+        q.m6 = ldg<NT>(chi_t + 6 * V, boff);
+        q.m7 = ldg<NT>(chi_t + 7 * V, boff);
+        q.m8 = ldg<NT>(chi_t + 8 * V, boff);
     }
 }
 
@@ -84,25 +113,47 @@ template <typename T, int NC, bool WITH_S> struct EpiIn {
     cx<T> s[WITH_S ? NC : 1];
 };
 
-template <typename T, int NC, int CK, bool WITH_S>
+// s_in: whether this voxel lies inside the source's support (only read when the source is restricted to a box;
+// outside it s is zero and is not loaded)
+template <bool NT, typename T, int NC, int CK, bool WITH_S>
 __device__ __forceinline__ void epi_load(EpiIn<T, NC, WITH_S>& in, const cx<T>* Et, const cx<T>* chit,
-                                         const cx<T>* st, int64_t V, uint32_t bo, int64_t tb) {
+                                         const cx<T>* st, int64_t V, uint32_t bo, int64_t tb, bool s_in = true) {
 #pragma unroll
-    for (int c = 0; c < NC; ++c) in.e[c] = at(Et + c * V, bo);
-    chi_load_ck<T, CK>(chit, V, bo, in.q, tb);
+    for (int c = 0; c < NC; ++c) in.e[c] = ldg<NT>(Et + c * V, bo);
+    chi_load_ck<NT, T, CK>(chit, V, bo, in.q, tb);
     if constexpr (WITH_S) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) in.s[c] = at(st + c * V, bo);
+        for (int c = 0; c < NC; ++c) in.s[c] = cx<T>{T(0), T(0)};
+        if (s_in) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) in.s[c] = ldg<NT>(st + c * V, bo);
+        }
     }
+}
+
+// Voxels of the update whose inputs (E, chi and the source) are requested ahead of their use, as a ring: the first
+// PRE voxels' loads are issued before the inverse FFT, and voxel m + PRE's when voxel m is taken out. Loads and stores
+// retire in issue order on this hardware, so with a distance of one voxel the update loop waited for one memory round
+// trip per voxel. Measured at 256^3 (three components, uniaxial chi, non-temporal traffic): 0.907 ms per iteration
+// with PRE = 1, 0.843 with 2, 0.835 with 4, 0.846 with 8 (all 8 voxels up front: more registers, fewer waves).
+#ifndef MMX_EPI_PREFETCH
+#define MMX_EPI_PREFETCH 0
+#endif
+template <int NC, int CK, int E>
+constexpr int epi_prefetch() {
+    if (MMX_EPI_PREFETCH > 0) return MMX_EPI_PREFETCH < E ? MMX_EPI_PREFETCH : E;
+    if (NC == 1) return E;
+    return (CK == 1 || CK == 4) ? (E < 4 ? E : 4) : 2;
 }
 
 // The host launches this kernel only when (N * inner + P) * sizeof(cx<T>) < 2^32, so every offset within a tile
 // fits the 32-bit byte offsets used below. CK is the chi kind of the passes that apply chi (0 for the others).
-// SB = 1 (update passes that also compute the next first pass, with a source confined to a box): s is added after
-// the update loop, only inside the box, instead of being loaded with every voxel's E and chi.
+// SB = 1 (update passes that also compute the next first pass, with a source confined to a box): s is loaded only
+// inside the box (masked loads), instead of with every voxel's E and chi.
 template <typename T, int N, int E, int KIND, int NC, int CK, int SB>
 __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
     using F = PassFlags<KIND>;
+    constexpr bool NT = nt_kind<KIND>();
     constexpr int TP = N / E;
     constexpr int LTP = ilog2(TP);
     extern __shared__ unsigned char smem_raw[];
@@ -158,9 +209,14 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         }
     }
 
-    load_twiddles<T, N, E>(tw, a.tw, tid, blockDim.x);
-    __syncthreads();
-
+    // the fused Green pass loads its twiddle table first (measured 173 against 180 us at 256^3); the other passes
+    // request it after their pencil loads, so that both arrive in one memory round trip (the table's LDS store waits
+    // for every load issued before it)
+    constexpr bool TW_FIRST = (KIND == kGreen);
+    if constexpr (TW_FIRST) {
+        load_twiddles<T, N, E>(tw, a.tw, tid, blockDim.x);
+        __syncthreads();
+    }
     cx<T> v[NC][E];
     const cx<T> zero{T(0), T(0)};
     // source support of this pencil: rows inside the box, points j in [s_jlo, s_jhi) (passes that add s are row
@@ -185,14 +241,14 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
             if (valid) {
                 const uint32_t bo = toff + m * mstep;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) x[c] = at(Et + c * V, bo);
+                for (int c = 0; c < NC; ++c) x[c] = ldg<NT>(Et + c * V, bo);
                 ChiVox<T> qv;
-                chi_load_ck<T, CK>(chit, V, bo, qv, tb);
+                chi_load_ck<NT, T, CK>(chit, V, bo, qv, tb);
                 chi_mul_ck<T, CK, NC>(qv, x, y, cf);
                 const int j = t + m * TP;
                 if (s_row && j >= s_jlo && j < s_jhi) {
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(at(st + c * V, bo), cf.a);
+                    for (int c = 0; c < NC; ++c) y[c] = y[c] + scal(ldg<NT>(st + c * V, bo), cf.a);
                 }
             }
 #pragma unroll
@@ -210,7 +266,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
             for (int m = 0; m < E; ++m) {
                 const uint32_t bo = toff + m * mstep;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) v[c][m] = at(srct + c * a.comp_stride, bo);
+                for (int c = 0; c < NC; ++c) v[c][m] = ldg<NT>(srct + c * a.comp_stride, bo);
             }
         }
     }
@@ -235,7 +291,13 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         for (int m = 0; m < E; ++m) kpre[m] = a.kvec[a.kofs_axis + t + m * TP];
     }
 
-    if constexpr (F::FWD_FIRST) reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    if constexpr (F::FWD_FIRST) {
+        if constexpr (!TW_FIRST) {
+            load_twiddles<T, N, E>(tw, a.tw, tid, blockDim.x);
+            __syncthreads();
+        }
+        reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    }
     if constexpr (F::GREEN) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -252,32 +314,44 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
             for (int c = 0; c < NC; ++c) v[c][m] = f[c];
         }
     }
-    // the update's inputs for the first voxel are requested before the inverse FFT, so that they arrive during it
-    constexpr bool S_INLINE = F::PRO_AFTER && SB == 0;
-    EpiIn<T, NC, S_INLINE> cur;
+    // the update's inputs for the first PRE voxels are requested before the inverse FFT, so that they arrive during
+    // it; a restricted source is read only inside its box (masked loads), a dense one with every voxel
+    constexpr bool S_IN = F::PRO_AFTER;
+    constexpr int PRE = F::EPI ? epi_prefetch<NC, CK, E>() : 1;
+    EpiIn<T, NC, S_IN> slot[PRE];
+    const int64_t V = a.comp_stride;
+    const cx<T>* Et = Ecur + b * a.E_bstride + tb;
+    const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
+    const cx<T>* st = a.s + b * a.s_bstride + tb;
+    auto s_inside = [&](int m) {
+        if constexpr (SB == 0) return true;
+        const int j = t + m * TP;
+        return s_row && j >= s_jlo && j < s_jhi;
+    };
     if constexpr (F::EPI) {
         if (valid) {
-            epi_load<T, NC, CK, S_INLINE>(cur, Ecur + b * a.E_bstride + tb, a.chi + b * a.chi_bstride + tb,
-                                          a.s + b * a.s_bstride + tb, a.comp_stride, toff, tb);
+#pragma unroll
+            for (int q = 0; q < PRE; ++q)
+                epi_load<NT, T, NC, CK, S_IN>(slot[q], Et, chit, st, V, toff + q * mstep, tb, s_inside(q));
         }
+    }
+    if constexpr (!F::FWD_FIRST && !TW_FIRST) {
+        load_twiddles<T, N, E>(tw, a.tw, tid, blockDim.x);
+        __syncthreads();
     }
     if constexpr (F::INV) reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
 
     if constexpr (F::EPI) {
-        const int64_t V = a.comp_stride;
-        const cx<T>* Et = Ecur + b * a.E_bstride + tb;
-        const cx<T>* chit = a.chi + b * a.chi_bstride + tb;
-        const cx<T>* st = a.s + b * a.s_bstride + tb;
         cx<T>* Eot = Enext + b * a.E_bstride + tb;
         double dd = 0.0, ee = 0.0;
         if (valid) {
-            // Loads and stores retire in issue order on this hardware, so every input of voxel m + 1 (E, chi and,
-            // for the fused next prologue, s) is requested before voxel m's result is stored (two voxels ahead
-            // measured no faster).
-            EpiIn<T, NC, S_INLINE> nxt;
 #pragma unroll
             for (int m = 0; m < E; ++m) {
-                if (m + 1 < E) epi_load<T, NC, CK, S_INLINE>(nxt, Et, chit, st, V, toff + (m + 1) * mstep, tb);
+                const EpiIn<T, NC, S_IN> cur = slot[m % PRE];
+                // refill the slot with voxel m + PRE before voxel m's stores are issued
+                if (m + PRE < E)
+                    epi_load<NT, T, NC, CK, S_IN>(slot[m % PRE], Et, chit, st, V, toff + (m + PRE) * mstep, tb,
+                                              s_inside(m + PRE));
                 const uint32_t bo = toff + m * mstep;
                 cx<T> w[3] = {zero, zero, zero}, d[3] = {zero, zero, zero}, en[3] = {zero, zero, zero};
 #pragma unroll
@@ -286,7 +360,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     en[c] = cur.e[c] + d[c];
-                    at(Eot + c * V, bo) = en[c];
+                    stg<NT>(Eot + c * V, bo, en[c]);
                     dd += norm2_f64(d[c]);
                     ee += norm2_f64(en[c]);
                 }
@@ -295,24 +369,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
                     cx<T> x[3] = {zero, zero, zero};
                     chi_mul_ck<T, CK, NC>(cur.q, en, x, cf);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        if constexpr (S_INLINE) v[c][m] = x[c] + scal(cur.s[c], cf.a);
-                        else v[c][m] = x[c];
-                    }
-                }
-                if (m + 1 < E) cur = nxt;
-            }
-            if constexpr (F::PRO_AFTER && SB != 0) {
-                if (s_row) {
-#pragma unroll
-                    for (int m = 0; m < E; ++m) {
-                        const int j = t + m * TP;
-                        if (j >= s_jlo && j < s_jhi) {
-#pragma unroll
-                            for (int c = 0; c < NC; ++c)
-                                v[c][m] = v[c][m] + scal(at(st + c * V, toff + m * mstep), cf.a);
-                        }
-                    }
+                    for (int c = 0; c < NC; ++c) v[c][m] = x[c] + scal(cur.s[c], cf.a);
                 }
             }
         }
@@ -326,7 +383,7 @@ __global__ void __launch_bounds__(512) reg_pass_kernel(PassArgs<T> a) {
         for (int m = 0; m < E; ++m) {
             const uint32_t bo = toff + m * mstep;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) at(dstt + c * a.comp_stride, bo) = scal(v[c][m], a.scale);
+            for (int c = 0; c < NC; ++c) stg<NT>(dstt + c * a.comp_stride, bo, scal(v[c][m], a.scale));
         }
       }
     }

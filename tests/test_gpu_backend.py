@@ -202,8 +202,10 @@ def test_load_selects_hip(device):
     assert isinstance(be, BackEndHip)
     be = backend.load(1, Grid([8, 8], 1.0), np.complex128, config_list=[{'type': 'numpy'}, {'type': 'hip'}])
     assert isinstance(be, BackEndHip) and not be.vectorial
+    be = backend.load(1, Grid([11, 8], 1.0), np.complex64)  # prime factor 11: Bluestein's algorithm
+    assert isinstance(be, BackEndHip)
     with pytest.raises(ValueError):
-        backend.load(1, Grid([11, 8], 1.0), np.complex64)  # prime factor 11: explicit error, no fallback
+        backend.load(1, Grid([8191, 8], 1.0), np.complex64)  # a prime above 2048: explicit error, no fallback
 
 
 def test_calc_roots_of_low_order_polynomial(BE):

@@ -16,7 +16,7 @@ def _plan(shape, dtype='c64'):
                 torch.device('cuda:0'))
 
 
-@pytest.mark.parametrize('shape', [(11,), (64, 13), (4097,), (8, 0, 4)])
+@pytest.mark.parametrize('shape', [(8191,), (64, 2 * 4099), (4096 * 4096 * 2,), (8, 0, 4)])
 def test_unsupported_or_invalid_shapes(shape, device):
     from macromax_amd._native import NativeError
     print(f'invalid shape {shape}', flush=True)

@@ -29,7 +29,9 @@ FFT_SHAPES = [(8,), (1024,), (2048,), (5,), (7,), (105,), (5, 10, 20), (64, 96),
               # lengths 5 * 2^k (register kernels with a radix-5 stage)
               (320,), (160, 80), (80, 160, 64), (64, 320, 80),
               # 640, 768 and 1024 points (register kernels with up to 128 threads per pencil)
-              (1024, 64), (768, 640), (64, 1024), (2048, 32), (16, 2048)]
+              (1024, 64), (768, 640), (64, 1024), (2048, 32), (16, 2048),
+              # prime factors above 7 (Bluestein's algorithm) and axes beyond one LDS tile (four-step split)
+              (11,), (101,), (64, 13), (13, 17, 11), (2039,), (8192,), (4097,), (2050, 3), (3, 12288), (24, 11, 40)]
 
 
 @pytest.mark.parametrize('shape', FFT_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
@@ -124,11 +126,14 @@ BORN_CASES = [
     ((80, 80, 80), 3, 4), ((160, 320), 1, 1), ((320, 80), 3, 3), ((80, 160, 64), 3, 2), ((320,), 3, 1),
     # 640, 768 and 1024 points
     ((1024, 256), 3, 3), ((640, 768), 1, 1), ((768, 128), 3, 4), ((64, 1024), 3, 2), ((2048, 64), 3, 3),
-    ((64, 2048), 1, 1),
+    ((64, 2048), 1, 1), ((64, 2048), 3, 4), ((32, 2048), 3, 2), ((2048,), 3, 1),
     # power-of-two axes (register-resident kernels) with every chi kind
     ((64, 128, 64), 3, 1), ((128, 64, 256), 3, 3), ((64, 64, 128), 1, 1), ((128, 256), 3, 3), ((256, 64), 3, 2),
     # uniaxial chi storage, LDS-staged and register kernels
     ((16, 20, 24), 3, 4), ((64, 48), 3, 4), ((128, 64, 256), 3, 4), ((128, 256), 3, 4),
+    # unfused iteration: prime factors above 7, four-step axes, fused tiles beyond LDS
+    ((13, 17, 11), 3, 2), ((101, 64), 1, 1), ((11, 24, 19), 3, 4), ((8192,), 1, 1), ((4097,), 3, 1),
+    ((4096,), 3, 1), ((2050, 6), 3, 3),
 ]
 
 
@@ -169,7 +174,8 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
 @pytest.mark.parametrize('shape,C,kind,shared', [((64, 48), 1, 1, True), ((12, 16, 20), 3, 2, True),
                                                  ((12, 16, 20), 3, 1, False), ((512,), 3, 2, False),
                                                  ((12, 16, 20), 3, 3, False), ((12, 16, 20), 3, 4, False),
-                                                 ((64, 32, 64), 3, 4, False)],
+                                                 ((64, 32, 64), 3, 4, False), ((13, 17, 11), 3, 2, False),
+                                                ((101, 24), 1, 1, True)],
                          ids=lambda v: str(v))
 def test_born_step_batched(shape, C, kind, shared, device):
     import torch
@@ -231,7 +237,12 @@ def test_longitudinal_projection_and_chi_mul(shape, device):
                                                 ((16, 20, 24), 3, 4, 'c128'), ((96, 96, 96), 3, 4, 'c64'),
                                                 ((192, 384), 1, 1, 'c64'), ((64, 96, 384), 3, 3, 'c64'),
                                                 ((80, 80, 80), 3, 4, 'c64'), ((160, 320), 1, 1, 'c64'),
-                                                ((1024, 256), 3, 3, 'c64'), ((768, 640), 1, 1, 'c64')],
+                                                ((1024, 256), 3, 3, 'c64'), ((768, 640), 1, 1, 'c64'),
+                                                ((64, 2048), 3, 4, 'c64'), ((32, 2048), 3, 2, 'c64'),
+                                                ((2048,), 3, 1, 'c64'), ((2048,), 1, 1, 'c64'),
+                                                ((13, 17, 11), 3, 3, 'c64'), ((13, 17, 11), 3, 2, 'c128'),
+                                                ((101, 64), 1, 1, 'c64'), ((8192,), 3, 1, 'c128'),
+                                                ((4096,), 3, 4, 'c128')],
                          ids=lambda v: str(v))
 def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     """mmx_born_prologue + mmx_born_iterate == mmx_born_step, and W afterwards == prologue(Eout)."""
