@@ -165,6 +165,21 @@ int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const 
                      double* norms, void* stream);
 
 /*
+ * One Born iteration for a magnetic or bi-(an)isotropic medium (MacroMax Solution.__iter__ with the curl-term
+ * susceptibility operator, solver.py:531-552 and 1064-1085; three components):
+ *     chi X = chi_EE X + chi_HE curl X + curl(chi_EH X + chi_HH curl X),  d = chi (G (chi E + s) - E),  Eout = E + d,
+ * curls with respect to the k0-scaled grid (i k x in k-space, backend/__init__.py:483-509), G the dyadic Green's
+ * operator for alpha = alpha[0] + i alpha[1]. chi[4] = {chi_EE, chi_HE, chi_EH, chi_HH}: chi_kind[i] 1 for a scalar
+ * field ([1,1,grid]), 2 for a 3x3 field ([3,3,grid] row-major); chi_vstride[i] 1 for a field, 0 for a constant (one
+ * value or one 3x3 matrix). E, Eout, s: [3,1,grid]; work: 12 * grid complex values of scratch (distinct from E and
+ * Eout); norms (device, 2 doubles): {sum |d|^2, sum |Eout|^2} in FP64. The caller applies the accept rule
+ * (solver.py:1078-1096) as for mmx_born_step.
+ */
+int mmx_magnetic_iterate(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* const* chi,
+                         const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,
+                         double* norms, void* stream);
+
+/*
  * Batched solves: nbatch problems of one grid shape (many right-hand sides of one medium when chi_bstride = 0, or
  * many media) iterate together with the accept/reject decision and the stop test taken on the device, so that a
  * run of calls can be captured in a graph and replayed without host synchronisation. No single reference

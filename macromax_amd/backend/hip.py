@@ -178,6 +178,41 @@ class Plan:
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
             raise ValueError('s size mismatch')
 
+    def magnetic_iterate(self, E, Eout, s, chis, alpha: complex, work, norms):
+        """One magnetic Born iteration (``mmx_magnetic_iterate``): Eout = E + d, norms = (|d|^2, |Eout|^2).
+
+        ``chis`` = (chi_EE, chi_HE, chi_EH, chi_HH), each a [1, 1, ...] scalar or [3, 3, ...] tensor field that is
+        either constant (one element per entry) or spans the whole grid (contiguous)."""
+        for t, w in ((E, 'E'), (Eout, 'Eout'), (s, 's'), (work, 'work')):
+            self._check(t, w)
+        if E.numel() != 3 * self.V or Eout.numel() != E.numel() or s.numel() != E.numel():
+            raise ValueError('magnetic_iterate: E, Eout and s must be [3, 1, *grid]')
+        if work.numel() < 12 * self.V:
+            raise ValueError('magnetic_iterate: the work buffer needs 12 grid-sized complex components')
+        if norms.dtype != torch.float64 or norms.numel() < 2 or norms.device != self.device:
+            raise ValueError('magnetic_iterate: norms must be a float64 device tensor of 2 elements')
+        ptrs, kinds, strides = [], [], []
+        for t in chis:
+            self._check(t, 'chi')
+            n = int(t.shape[0])
+            if n not in (1, 3) or int(t.shape[1]) != n:
+                raise ValueError('magnetic_iterate: susceptibilities are [1, 1, ...] or [3, 3, ...] fields')
+            per = t.numel() // (n * n)
+            if per not in (1, self.V):
+                raise ValueError('magnetic_iterate: a susceptibility must be constant or span the grid')
+            ptrs.append(t.data_ptr())
+            kinds.append(1 if n == 1 else 2)
+            strides.append(0 if per == 1 and self.V > 1 else 1)
+        c_p = (ctypes.c_void_p * 4)(*ptrs)
+        c_k = (ctypes.c_int * 4)(*kinds)
+        c_s = (ctypes.c_int64 * 4)(*strides)
+        c_a = (ctypes.c_double * 2)(float(alpha.real), float(alpha.imag))
+        with torch.cuda.device(self.device):
+            _native.check(self._lib.mmx_magnetic_iterate(self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(),
+                                                         c_p, c_k, c_s, c_a, work.data_ptr(), norms.data_ptr(),
+                                                         _stream_ptr(self.device)), 'mmx_magnetic_iterate')
+        return norms
+
     def born_prologue(self, E, W, s, chi, ncomp, chi_kind, coef, nbatch=1, s_bstride=0, chi_bstride=0,
                       s_box=None):
         """W = FFT along the innermost axis of chi_eff E + s_eff (``mmx_born_prologue``)."""

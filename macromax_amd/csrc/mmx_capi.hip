@@ -11,6 +11,7 @@
 
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
+#include "mmx_magnetic.hpp"
 #include "mmx_pointwise.hpp"
 #include "mmx_regkernels.hpp"
 #include "mmx_setup.hpp"
@@ -1114,6 +1115,76 @@ int batch_step_impl(mmx_plan_s* p, void* EA, void* EB, void* W, const void* s, c
     return MMX_OK;
 }
 
+// One magnetic Born iteration (see mmx_magnetic.hpp for the sequence).
+template <typename T>
+int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* const* chi, const int* kinds,
+                  const int64_t* vstride, const double* alpha, void* work, double* norms, hipStream_t st) {
+    if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
+    MagArgs<T> a;
+    memset(&a, 0, sizeof(a));
+    MagChi<T>* m[4] = {&a.ee, &a.he, &a.eh, &a.hh};
+    for (int i = 0; i < 4; ++i) {
+        if (!chi[i]) return fail(MMX_EINVAL, "null susceptibility tensor");
+        if (kinds[i] != 1 && kinds[i] != 2) return fail(MMX_EINVAL, "susceptibility kinds are 1 (scalar) or 2 (3x3)");
+        if (vstride[i] != 0 && vstride[i] != 1) return fail(MMX_EINVAL, "voxel strides are 0 or 1");
+        m[i]->p = reinterpret_cast<const cx<T>*>(chi[i]);
+        m[i]->kind = kinds[i];
+        m[i]->vstride = vstride[i];
+    }
+    const int64_t V = p->V;
+    cx<T>* w = reinterpret_cast<cx<T>*>(work);
+    a.E = reinterpret_cast<const cx<T>*>(E);
+    a.Eout = reinterpret_cast<cx<T>*>(Eout);
+    a.s = reinterpret_cast<const cx<T>*>(s);
+    a.w1 = w;
+    a.w2 = w + 3 * V;
+    a.w3 = w + 6 * V;
+    a.V = V;
+    a.n1 = p->shape3[1];
+    a.n2 = p->shape3[2];
+    a.fn1 = make_fdiv64(p->shape3[1]);
+    a.fn2 = make_fdiv64(p->shape3[2]);
+    a.kvec = reinterpret_cast<const T*>(p->kvec);
+    a.k1ofs = p->k1ofs;
+    a.k2ofs = p->k2ofs;
+    a.nd = p->ndim;
+    a.alpha_r = T(alpha[0]);
+    a.alpha_i = T(alpha[1]);
+    const int nblk = pointwise_blocks(V);
+    int rc = ensure_scratch(p, (size_t)(2 * nblk) * sizeof(double));
+    if (rc) return rc;
+    a.partials = reinterpret_cast<double*>(p->scratch);
+    const dim3 grid(nblk), blk(256);
+    auto check = [&]() -> int {
+        MMX_HIP_CHECK(hipGetLastError());
+        return MMX_OK;
+    };
+    // E^ and curl E
+    if ((rc = fft_passes<T>(p, E, a.w1, 3, -1, true, st))) return rc;
+    hipLaunchKernelGGL((cross_ik_kernel<T>), grid, blk, 0, st, a, a.w1, a.w2);
+    if ((rc = check())) return rc;
+    if ((rc = fft_passes<T>(p, a.w2, a.w2, 3, +1, true, st))) return rc;
+    // [Z1; Y1], their transforms, B^ and i k x B^, back to [B; curl B]
+    hipLaunchKernelGGL((magnetic_real_kernel<T>), grid, blk, 0, st, a, a.E, a.w2, 1);
+    if ((rc = check())) return rc;
+    if ((rc = fft_passes<T>(p, a.w3, a.w3, 6, -1, true, st))) return rc;
+    hipLaunchKernelGGL((magnetic_green_kernel<T>), grid, blk, 0, st, a);
+    if ((rc = check())) return rc;
+    if ((rc = fft_passes<T>(p, a.w3, a.w3, 6, +1, true, st))) return rc;
+    // [Z2; Y2] and curl Y2
+    hipLaunchKernelGGL((magnetic_real_kernel<T>), grid, blk, 0, st, a, a.w3, a.w3 + 3 * V, 0);
+    if ((rc = check())) return rc;
+    cx<T>* y2 = a.w3 + 3 * V;
+    if ((rc = fft_passes<T>(p, y2, y2, 3, -1, true, st))) return rc;
+    hipLaunchKernelGGL((cross_ik_kernel<T>), grid, blk, 0, st, a, y2, y2);
+    if ((rc = check())) return rc;
+    if ((rc = fft_passes<T>(p, y2, y2, 3, +1, true, st))) return rc;
+    // d = Z2 + curl Y2, E' and the norms
+    hipLaunchKernelGGL((magnetic_final_kernel<T>), grid, blk, 0, st, a);
+    if ((rc = check())) return rc;
+    return reduce_norms(a.partials, nblk, 1, norms, st);
+}
+
 // Device address of `scratch` on the plan's device, looked up once per device (with that device current) and kept.
 constexpr int kMaxDevices = 64;
 std::mutex g_noalloc_mutex;
@@ -1393,6 +1464,19 @@ int mmx_born_step_noalloc(mmx_plan_t p, const void* E, void* Eout, const void* s
                                   partials, cap)
                : born_impl<double>(p, E, Eout, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
                                    st, partials, cap);
+}
+
+int mmx_magnetic_iterate(mmx_plan_t p, const void* E, void* Eout, const void* s, const void* const* chi,
+                         const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,
+                         double* norms, void* stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !s || !chi || !chi_kind || !chi_vstride || !alpha || !work || !norms)
+        return fail(MMX_EINVAL, "null argument");
+    if (E == Eout || E == work || Eout == work) return fail(MMX_EINVAL, "E, Eout and work must be different buffers");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    return p->dtype == MMX_C64
+               ? magnetic_impl<float>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st)
+               : magnetic_impl<double>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st);
 }
 
 int mmx_born_prologue(mmx_plan_t p, const void* E, void* W, const void* s, const void* chi, int ncomp, int chi_kind,

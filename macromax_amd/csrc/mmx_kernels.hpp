@@ -106,22 +106,35 @@ struct PassArgs {
 template <typename T> __device__ __forceinline__ T recip(T x) { return T(1) / x; }
 template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// g = -i alpha_i / (k^2 - alpha) = (1 - i u) / (1 + u^2) with u = (k^2 - alpha_r) / alpha_i, and the longitudinal
+// i alpha_i / alpha = (1 + i v) / (1 + v^2) with v = alpha_r / alpha_i: no alpha_i^2 is formed, which overflows
+// single precision for the alpha_i of 1e19 and more that the preconditioner search can give (a magnetic medium on a
+// fine grid, where beta tends to 0).
+template <typename T>
+__device__ __forceinline__ cx<T> green_scalar(T kk, T alpha_r, T inv_ai, T scale) {
+    const T u = (kk - alpha_r) * inv_ai;
+    const T r = scale * recip(T(1) + u * u);
+    return cx<T>{r, -u * r};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> green_longitudinal(T alpha_r, T inv_ai, T scale) {
+    const T v = alpha_r * inv_ai;
+    const T r = scale / (T(1) + v * v);
+    return cx<T>{r, v * r};
+}
+
 template <typename T>
 __device__ __forceinline__ void green_apply(cx<T>* f, int C, T k0, T k1, T k2, bool dc, const PassArgs<T>& a,
                                             const Coef<T>& cf) {
     const T kk = k0 * k0 + k1 * k1 + k2 * k2;
-    // g = -i alpha_i / (kk - alpha) / N = alpha_i (alpha_i - i x) / (x^2 + alpha_i^2) / N with x = kk - alpha_r
-    const T x = kk - cf.alpha_r;
-    const T r = cf.alpha_i * a.inv_ntotal * recip(x * x + cf.alpha_i * cf.alpha_i);
-    const cx<T> g{cf.alpha_i * r, -x * r};
+    const T inv_ai = T(1) / cf.alpha_i;  // uniform: computed once per tile
+    const cx<T> g = green_scalar(kk, cf.alpha_r, inv_ai, a.inv_ntotal);
     if (!a.vectorial || C == 1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) if (c < C) f[c] = g * f[c];
         return;
     }
-    // i alpha_i / alpha / N = alpha_i (alpha_i + i alpha_r) / |alpha|^2 / N
-    const T rl = cf.alpha_i * a.inv_ntotal / (cf.alpha_r * cf.alpha_r + cf.alpha_i * cf.alpha_i);
-    const cx<T> gl{cf.alpha_i * rl, cf.alpha_r * rl};
+    const cx<T> gl = green_longitudinal(cf.alpha_r, inv_ai, a.inv_ntotal);
     if (dc) {  // k = 0: the first nd_data components count as longitudinal
 #pragma unroll
         for (int c = 0; c < 3; ++c) f[c] = (c < a.nd_data ? gl : g) * f[c];

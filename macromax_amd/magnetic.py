@@ -12,9 +12,9 @@ With permeability mu (not a constant), or coupling tensors xi, zeta, the suscept
 and alpha_r, beta chosen by the reference's Nelder-Mead search over (alpha_r, sqrt(beta)) of
 beta * (sigma_D^2 sigma_HH(beta) + sigma_D (sigma_EH + sigma_HE)/beta + sigma_EE(alpha_r, beta)) (``solver.py:405-443``).
 The largest singular and eigen values over the volume come from this package's FP64 set-up reduction kernel
-(``mmx_setup_sigma``: max over voxels of the largest eigenvalue of (M - shift I)^H (M - shift I)). Each iteration
-runs as HIP FFTs (``mmx_fft``: four curls and one Green's function per iteration), the HIP 3x3 multiply
-(``mmx_chi_mul``) and device element-wise operations; the norms are FP64 device reductions.
+(``mmx_setup_sigma``: max over voxels of the largest eigenvalue of (M - shift I)^H (M - shift I)). Each iteration is
+one ``mmx_magnetic_iterate`` call: the FFTs of 24 field components and five voxel-wise kernels, with chi E + s and
+G (chi E + s) kept in k-space, and FP64 norms.
 """
 from __future__ import annotations
 
@@ -108,6 +108,8 @@ class MagneticOperator:
         self.chiHE_beta = prep(chiHE_beta)
         self.s = None
         self._g = None  # (alpha, g) of the Green's operator
+        self._ops = None  # (version, susceptibilities as the fused kernels take them)
+        self._work = None  # 12 grid-sized components of scratch for mmx_magnetic_iterate
         self.version = 0  # incremented whenever s, alpha or a susceptibility tensor is replaced
 
     # ------------------------------------------------------------------ operators
@@ -152,8 +154,25 @@ class MagneticOperator:
         return BE.ift(F)
 
     def update(self, E: torch.Tensor) -> torch.Tensor:
-        """d = chi (G (chi E + s) - E)."""
+        """d = chi (G (chi E + s) - E), composed of the backend's curl, Green's and element-wise operations."""
         return self.chi_op(self.green(self.chi_op(E) + self.s) - E)
+
+    def _kernel_operand(self, t: torch.Tensor) -> torch.Tensor:
+        """A susceptibility as mmx_magnetic_iterate takes it: constant (one element per entry) or spanning the grid."""
+        n = int(t.shape[0])
+        if t.numel() == n * n:
+            return t.reshape(n, n).contiguous()
+        return _full(t, self.shape)
+
+    def iterate(self, E: torch.Tensor, Eout: torch.Tensor, norms: torch.Tensor):
+        """Eout = E + d, norms = (|d|^2, |Eout|^2) in FP64 with the fused kernel sequence (mmx_magnetic_iterate:
+        24 transformed components and five voxel-wise kernels per iteration)."""
+        if self._ops is None or self._ops[0] != self.version:
+            self._ops = (self.version, [self._kernel_operand(t) for t in (self.chiEE, self.chiHE, self.chiEH,
+                                                                          self.chiHH)])
+        if self._work is None:
+            self._work = torch.empty((12, *self.shape), dtype=self.BE.hardware_dtype, device=self.BE.device)
+        self.plan.magnetic_iterate(E, Eout, self.s, self._ops[1], self.alpha, self._work, norms)
 
     def set_alpha(self, alpha_new: complex):
         """``solver.py:496-529`` for a new alpha (after a rejected update)."""

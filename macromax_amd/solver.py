@@ -670,12 +670,7 @@ class Solution:
     def __magnetic_compute(self, cur: int, nxt: int, norms: torch.Tensor):
         """d = chi (G (chi E + s) - E) for E in field buffer cur; buffer nxt = E + d; norms = (|d|^2, |E + d|^2) in
         FP64 (``solver.py:1070-1082``)."""
-        E = self.__fields[cur]
-        d = self.__mag.update(E).contiguous()
-        torch.add(E, d, out=self.__fields[nxt])
-        plan = self.__BE.plan
-        plan.norm2(d, norms[0:1])
-        plan.norm2(self.__fields[nxt], norms[1:2])
+        self.__mag.iterate(self.__fields[cur], self.__fields[nxt], norms)
 
     def __magnetic_launch(self, cur: int, nxt: int) -> torch.Tensor:
         """Enqueue one magnetic iteration; returns the device tensor that will hold its two squared norms.

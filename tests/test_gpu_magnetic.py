@@ -226,3 +226,93 @@ def test_magnetic_graphs_follow_a_reallocated_plan_scratch():
         graphed.step()
         npt.assert_array_equal(graphed.field_array.cpu().numpy(), eager.field_array.cpu().numpy(), err_msg=str(it))
         assert graphed.residue == eager.residue
+
+
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128], ids=['c64', 'c128'])
+def test_magnetic_3d_tensor_media_match_oracle(dtype):
+    """3D medium with a constant anisotropic permeability tensor and a bi-isotropy xi per voxel, so the fused
+    iteration (mmx_magnetic_iterate) sees a constant 3x3 chi_HH, 3x3 fields chi_EH and chi_HE and a scalar chi_EE:
+    10 iterates against MagneticBornOracle (relative L2 <= 1e-10 complex128, <= 2e-5 complex64), and the first update
+    against the composed operator (MagneticOperator.update: curls, Green's operator and element-wise products)."""
+    import scipy.constants as const
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    print(f'magnetic 3d tensors {np.dtype(dtype).name}', flush=True)
+    rng = np.random.default_rng(12)
+    shape = (12, 10, 16)
+    pitch, wavelength = 0.1e-6, 0.5e-6
+    k0 = 2 * np.pi / wavelength
+    grid = mm.Grid(list(shape), pitch)
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2
+    m = rng.standard_normal((3, 3)) * 0.1
+    mu = (np.eye(3) * 1.2 + m @ m.T).reshape(3, 3, 1, 1, 1)
+    xi = (0.05 * (rng.random(shape) + 1j * rng.random(shape)))
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[0, :, :, 4] = 1.0
+    J[1, :, :, 4] = 0.3j
+    bound = LinearBound(grid, thickness=0.3e-6)
+    sol = mm.Solution(grid, vacuum_wavelength=wavelength, current_density=J, epsilon=eps, mu=mu, xi=xi,
+                      bound=bound, dtype=dtype)
+    assert sol.magnetic
+    mag = sol._Solution__mag
+    E0 = torch.zeros((3, 1, *shape), dtype=mag.s.dtype, device=mag.s.device)
+    d_composed = mag.update(E0)
+    E1 = torch.empty_like(E0)
+    norms = torch.zeros(2, dtype=torch.float64, device=E0.device)
+    mag.iterate(E0, E1, norms)
+    rel = float(torch.linalg.vector_norm(E1 - d_composed) / torch.linalg.vector_norm(d_composed))
+    assert rel <= (1e-12 if dtype == np.complex128 else 2e-6), rel
+    npt.assert_allclose(norms[0].item(), float(torch.linalg.vector_norm(d_composed)) ** 2, rtol=1e-5)
+    eps_b = eps + bound.electric_susceptibility
+    alpha, beta = sol.alpha, sol.beta
+    _, _, ops, _, _ = ob.magnetic_setup(shape, pitch, k0, eps_b, xi, 0.0, mu, np.asarray(bound.thickness).ravel(),
+                                        alpha_beta=(alpha, beta))
+    S = J.reshape(3, 1, *shape) * (-1j * k0 * const.c * const.mu_0)
+    k = ob.k_vectors(shape, np.array([pitch] * 3) * k0)
+    o = ob.MagneticBornOracle(ops, S * (1j / alpha.imag / beta), alpha, beta, k)
+    tol = 1e-10 if dtype == np.complex128 else 2e-5
+    for it in range(10):
+        sol.step()
+        o.step()
+        assert sol.alpha == o.alpha
+        Eg = sol.field_array.cpu().numpy()
+        err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
+        assert err <= tol, (it, err)
+
+
+def test_magnetic_fine_grid_extreme_alpha_matches_oracle():
+    """A negative-index plate on a fine grid (pitch lambda / 16): the reference's two-parameter search (restated by
+    the oracle) drives beta towards 0 and alpha_i to about 4.5e19, whose square overflows single precision. The
+    complex64 Green's operator is evaluated without alpha_i^2; 10 iterates against MagneticBornOracle within 2e-5."""
+    import scipy.constants as const
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    print('magnetic plate at lambda / 16, extreme alpha', flush=True)
+    wavelength = 500e-9
+    k0 = 2 * np.pi / wavelength
+    size, pitch = 64, wavelength / 16
+    grid = mm.Grid([size, size], pitch)
+    nidx = 1 + (-1.5 - 1) * np.ones(grid[1].shape) * (np.abs(grid[0]) < 0.5e-6)
+    J = np.zeros((3, size, size), dtype=np.complex128)
+    J[1, 8, :] = np.exp(-((np.arange(size) - 32) / 10.0) ** 2)
+    J[2, 8, :] = 0.5j * J[1, 8, :]
+    bound = LinearBound(grid, thickness=0.5e-6, max_extinction_coefficient=0.50)
+    sol = mm.Solution(grid, vacuum_wavelength=wavelength, current_density=J, refractive_index=nidx, bound=bound,
+                      dtype=np.complex64)
+    assert sol.magnetic and sol.alpha.imag > 1e18, sol.alpha
+    sign = 1 - 2 * (nidx < 0)
+    eps_b = (nidx ** 2 * sign)[None, None] + bound.electric_susceptibility
+    _, _, ops, _, _ = ob.magnetic_setup((size, size), pitch, k0, eps_b, 0.0, 0.0, sign[None, None].astype(complex),
+                                        np.asarray(bound.thickness).ravel(), alpha_beta=(sol.alpha, sol.beta))
+    S = J.reshape(3, 1, size, size) * (-1j * k0 * const.c * const.mu_0)
+    k = ob.k_vectors((size, size), np.array([pitch, pitch]) * k0)
+    o = ob.MagneticBornOracle(ops, S * (1j / sol.alpha.imag / sol.beta), sol.alpha, sol.beta, k)
+    for it in range(10):
+        sol.step()
+        o.step()
+        Eg = sol.field_array.cpu().numpy()
+        err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
+        assert err <= 2e-5, (it, err)
