@@ -499,9 +499,9 @@ def magnetic_benchmark(n=512, log=lambda m: None):
         rec['cpu_oracle_extrapolated_s'] = round(rec['cpu_oracle_seconds_per_iteration'] * sol.iteration, 1)
         out[f'{size}'] = rec
         del sol
-    out['timed'] = ('Solution.solve to residue 1e-3 (set-up excluded); the magnetic iteration is HIP FFT curls, the '
-                    'HIP 3x3 multiply and element-wise device operations (not fused); CPU: one complex128 oracle '
-                    'iteration (numpy.fft, 1 core) x the GPU iteration count')
+    out['timed'] = ('Solution.solve to residue 1e-3 (set-up excluded); one magnetic iteration is mmx_magnetic_iterate '
+                    '(the FFTs of 24 field components and five voxel-wise kernels) replayed from a HIP graph; CPU: one '
+                    'complex128 oracle iteration (numpy.fft, 1 core) x the GPU iteration count')
     return out
 
 
