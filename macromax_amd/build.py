@@ -14,8 +14,8 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB_NAME = 'libmacromax_hip.so'
 LIB_PATH = os.path.join(HERE, LIB_NAME)
 SOURCES = ['mmx_capi.hip']
-HEADERS = ['mmx_device.hpp', 'mmx_kernels.hpp', 'mmx_pointwise.hpp', 'mmx_regfft.hpp', 'mmx_regkernels.hpp',
-           'mmx_setup.hpp']
+HEADERS = ['mmx_plan.hpp', 'mmx_device.hpp', 'mmx_kernels.hpp', 'mmx_magnetic.hpp', 'mmx_pointwise.hpp', 'mmx_regfft.hpp',
+           'mmx_regkernels.hpp', 'mmx_setup.hpp']
 ARCH = os.environ.get('MMX_OFFLOAD_ARCH', 'gfx950')
 
 

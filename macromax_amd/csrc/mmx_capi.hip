@@ -11,6 +11,7 @@
 
 #include "../../include/macromax_hip.h"
 #include "mmx_kernels.hpp"
+#include "mmx_plan.hpp"
 #include "mmx_magnetic.hpp"
 #include "mmx_pointwise.hpp"
 #include "mmx_regkernels.hpp"
@@ -25,44 +26,11 @@ static __device__ __align__(256) double scratch[MMX_NOALLOC_SCRATCH_BYTES / size
 
 namespace {
 
-thread_local std::string g_last_error;
-
-int fail(int code, const std::string& msg) {
-    g_last_error = msg;
-    return code;
-}
-
 #define MMX_HIP_CHECK(expr)                                                                    \
     do {                                                                                       \
         hipError_t e_ = (expr);                                                                \
         if (e_ != hipSuccess) return fail(MMX_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
-
-constexpr int kMaxN = 4096;          // longest axis a single pass holds in LDS
-constexpr size_t kLdsBudget = 65536; // preferred dynamic LDS per workgroup (more only when one pencil needs it)
-constexpr int kElemsF = 8;           // complex elements per thread per Stockham pass (complex64)
-constexpr int kElemsD = 4;           // (complex128)
-// Larger per-thread budgets, used only when a tile of one long pencil would need more than 1024 threads.
-constexpr int kElemsFBig = 16;
-constexpr int kElemsDBig = 8;
-
-struct AxisPlan {
-    int orig;        // original grid axis (0..2)
-    int N;           // points along the axis
-    int NT;          // length of the FFTs that transform it in one pass: N, or the Bluestein length (> 2N - 2)
-    int nrad;
-    int rad[kMaxRadices];  // radices of NT
-    int64_t inner;   // stride of the axis
-    int64_t nouter;  // blocks before the axis (per field)
-    void* tw;        // device twiddles (forward), NT complex
-    int blue;        // 1: Bluestein's algorithm (N has a prime factor above 7)
-    void* chirp;     // N complex exp(-i pi n^2 / N)
-    void* bhat_f;    // NT complex: FFT of the wrapped conjugate chirp / NT (forward transforms)
-    void* bhat_i;    // the same for the inverse transform (the chirp itself)
-    int split;       // 1: four-step: the axis is N1 x N2 sub-axes (n = n1 N2 + n2), each transformed in its own pass
-    int N1, N2;
-    void* tw4;       // N complex exp(-2 pi i t / N), the four-step twiddles w^(k1 n2)
-};
 
 }  // namespace
 
@@ -97,57 +65,6 @@ struct mmx_plan_s {
 };
 
 namespace {
-
-bool factorize(int n, int* rad, int& nrad) {
-    static const int pref[] = {8, 4, 2, 3, 5, 7};
-    nrad = 0;
-    int m = n;
-    for (int r : pref) {
-        while (m % r == 0 && m > 1) {
-            if (nrad >= kMaxRadices) return false;
-            rad[nrad++] = r;
-            m /= r;
-        }
-    }
-    return m == 1;
-}
-
-bool smooth7(int64_t n) {
-    if (n < 1) return false;
-    for (int r : {2, 3, 5, 7})
-        while (n % r == 0) n /= r;
-    return n == 1;
-}
-
-int64_t next_smooth7(int64_t n) {
-    while (!smooth7(n)) ++n;
-    return n;
-}
-
-// One pass can transform an axis of n points directly (Stockham, n 7-smooth) or by Bluestein's algorithm (a 7-smooth
-// transform length >= 2n - 1), both at most kMaxN points in LDS.
-bool direct_ok(int64_t n) { return n <= kMaxN && smooth7(n); }
-bool blue_ok(int64_t n) { return n >= 2 && next_smooth7(2 * n - 1) <= kMaxN; }
-
-// Four-step split n = n1 n2 with both sub-lengths transformable in one pass: both 7-smooth if possible, then the
-// most balanced. Returns false when none exists (a prime factor above kMaxN / 2, or n beyond kMaxN^2).
-bool choose_split(int64_t n, int& n1, int& n2) {
-    int best_score = 1 << 30;
-    for (int64_t d = 2; d * d <= n; ++d) {
-        if (n % d) continue;
-        const int64_t e = n / d;
-        if (e > kMaxN) continue;
-        const bool ok_d = direct_ok(d) || blue_ok(d), ok_e = direct_ok(e) || blue_ok(e);
-        if (!ok_d || !ok_e) continue;
-        const int score = (direct_ok(d) ? 0 : 1 << 20) + (direct_ok(e) ? 0 : 1 << 20) + (int)(e - d);
-        if (score < best_score) {
-            best_score = score;
-            n1 = (int)d;
-            n2 = (int)e;
-        }
-    }
-    return best_score < (1 << 30);
-}
 
 template <typename T>
 int upload_complex(const std::vector<std::complex<double>>& h, void** dev) {
@@ -206,24 +123,6 @@ int upload_real(const std::vector<double>& h, void** dev) {
     return MMX_OK;
 }
 
-// k-vectors 2 pi fftfreq(n, d) in the order of numpy.fft.fftfreq (utils/ft/grid.py:426-439). For the transposed
-// operator the Nyquist entry -pi/d of every even axis becomes +pi/d: A = F^-1 D(k) F has A^T = F^-1 D(k[-n]) F, and
-// k[-n] = -k[n] everywhere except at n = N/2, where the dyadic Green's cross terms change sign.
-std::vector<double> k_vectors(const mmx_plan_s* p) {
-    std::vector<double> k;
-    for (int a = 0; a < 3; ++a) {
-        const int64_t n = p->shape3[a];
-        const double val = 1.0 / (double(n) * p->step3[a]);
-        const int64_t npos = (n - 1) / 2 + 1;
-        for (int64_t i = 0; i < n; ++i) {
-            double f = (i < npos ? double(i) : double(i - n)) * val;
-            if (p->transposed && n % 2 == 0 && n > 1 && i == n / 2) f = -f;
-            k.push_back(2.0 * M_PI * f);
-        }
-    }
-    return k;
-}
-
 template <typename T>
 int reupload_real(const std::vector<double>& h, void* dev) {
     std::vector<T> tmp(h.begin(), h.end());
@@ -278,79 +177,6 @@ int ensure_scratch(mmx_plan_s* p, size_t bytes) {
     return MMX_OK;
 }
 
-int pow2floor(int64_t x) {
-    int r = 1;
-    while ((int64_t)r * 2 <= x) r *= 2;
-    return r;
-}
-
-// Tile shape for one pass: pencils per tile P, threads, dynamic LDS bytes, tiles per problem.
-struct TileCfg {
-    int P, nth, pad;
-    int big;                    // 1: the kernel variant with the larger per-thread element budget
-    size_t lds;
-    int64_t nouter;             // outer blocks the pass iterates (folded passes include batch and components)
-    int64_t ntile_inner;
-    int64_t tiles_per_problem;
-    int64_t nblocks;
-};
-
-// fused: tiles hold `ncomp` components of one problem, grid = tiles_per_problem * nbatch.
-// plain: (batch, component, outer) folded into nouter, ncomp = 1, `nbatch` counts fields.
-template <typename T>
-int tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, size_t lds_max, TileCfg& t,
-                int p_request = 0, size_t lds_budget = kLdsBudget, bool want_big = false) {
-    int E = sizeof(T) == 4 ? kElemsF : kElemsD;
-    t.big = 0;
-    if (want_big || (int64_t)(fused ? ncomp : 1) * ax.NT > (int64_t)1024 * E) {
-        E = sizeof(T) == 4 ? kElemsFBig : kElemsDBig;
-        t.big = 1;
-    }
-    const size_t el = sizeof(cx<T>);
-    const int N = ax.NT;  // points per pencil in LDS
-    const bool rows = ax.inner == 1;
-    if (!fused) ncomp = 1;
-    t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
-    const int64_t mult = fused ? nbatch : 1;  // independent tile groups besides nouter
-    t.pad = rows ? 1 : 0;
-    const size_t per = (size_t)ncomp * (size_t)(N + t.pad) * el;
-    size_t budget = lds_budget > lds_max ? lds_max : lds_budget;
-    if (N * el + per > budget) budget = lds_max;
-    if (N * el + per > budget) return fail(MMX_EUNSUP, "FFT length too large for one LDS tile");
-    int64_t pm = (int64_t)((budget - N * el) / per);
-    const int64_t pmax_thr = (int64_t)1024 * E / ((int64_t)ncomp * N);
-    if (pmax_thr < 1) return fail(MMX_EUNSUP, "FFT length too large for one workgroup");
-    if (pm > pmax_thr) pm = pmax_thr;
-    if (rows) {
-        const int64_t total_rows = t.nouter * mult;
-        int64_t target = 8;
-        const int64_t by_tiles = total_rows / 1024;  // keep >= ~1024 tiles when the field allows it
-        if (by_tiles < target) target = by_tiles < 1 ? 1 : by_tiles;
-        if (p_request > 0) target = p_request;
-        if (pm > target) pm = target;
-        if (pm > t.nouter) pm = t.nouter;
-        t.P = p_request > 0 ? (int)(pm < 1 ? 1 : pm) : pow2floor(pm < 1 ? 1 : pm);
-        t.ntile_inner = 1;
-        t.tiles_per_problem = (t.nouter + t.P - 1) / t.P;
-    } else {
-        const int64_t target = p_request > 0 ? p_request : 16;
-        if (pm > target) pm = target;
-        t.P = ax.inner < pm ? (int)ax.inner : (p_request > 0 ? (int)(pm < 1 ? 1 : pm) : pow2floor(pm < 1 ? 1 : pm));
-        // few tiles in total: trade segment length for parallelism
-        if (p_request <= 0)
-            while (t.P > 2 && t.nouter * mult * ((ax.inner + t.P - 1) / t.P) < 512) t.P /= 2;
-        t.ntile_inner = (ax.inner + t.P - 1) / t.P;
-        t.tiles_per_problem = t.nouter * t.ntile_inner;
-    }
-    const int64_t need = ((int64_t)ncomp * t.P * N + E - 1) / E;
-    t.nth = (int)(((need + 63) / 64) * 64);
-    if (t.nth < 64) t.nth = 64;
-    if (t.nth > 1024) return fail(MMX_EUNSUP, "internal: tile needs more than 1024 threads");
-    t.lds = (size_t)N * el + (size_t)ncomp * t.P * (N + t.pad) * el;
-    t.nblocks = t.tiles_per_problem * mult;
-    return MMX_OK;
-}
-
 template <typename T>
 void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.N = ax.N;
@@ -383,39 +209,6 @@ void fill_geometry(const mmx_plan_s* p, const AxisPlan& ax, PassArgs<T>& a) {
     a.axis_orig = ax.orig;
     a.kofs_axis = ax.orig == 0 ? 0 : (ax.orig == 1 ? p->k1ofs : p->k2ofs);
 }
-
-// ---- register-resident kernels (complex64, N = 2^k, 3 * 2^k or 5 * 2^k, 64 <= N <= 2048) ----
-// points per thread: fewer for the fused last pass, which also holds E, chi and s of its voxels; lengths with a
-// factor 3 use 6 points per thread for three components (radix-2 stages, then radix 3; 132-172 VGPRs, measured 2-6 %
-// faster than 12 points with 192-225 VGPRs) and 24 for one; lengths with a factor 5 use 10 and 20
-#ifndef MMX_E_FUSED3
-#define MMX_E_FUSED3 8
-#endif
-#ifndef MMX_E_GREEN3
-#define MMX_E_GREEN3 8
-#endif
-#ifndef MMX_E_RADIX3_C3
-#define MMX_E_RADIX3_C3 6
-#endif
-constexpr int reg_elems(int N, int NC, int KIND) {
-    return (N % 5 == 0)                              ? (NC == 3 ? 10 : 20)
-           : (N % 3 == 0)                              ? (NC == 3 ? MMX_E_RADIX3_C3 : 24)
-           : (KIND == kEpiPro || KIND == kGreenEpiPro) ? (NC == 3 ? MMX_E_FUSED3 : 8)
-           : (KIND == kGreen && NC == 3)             ? MMX_E_GREEN3
-           : (NC == 3 ? 8 : (N >= 256 ? 16 : 8));
-}
-
-// Lengths the register kernels are instantiated for; their in-tile byte offsets are 32-bit, so the strided extent
-// N * inner (plus at most 512 pencils) must stay below 2^32 bytes.
-bool reg_supported(const AxisPlan& ax) {
-    if (ax.blue || ax.split || ax.NT != ax.N) return false;
-    const bool len = ax.N == 64 || ax.N == 128 || ax.N == 256 || ax.N == 512 || ax.N == 96 || ax.N == 192 ||
-                     ax.N == 384 || ax.N == 80 || ax.N == 160 || ax.N == 320 || ax.N == 640 || ax.N == 768 ||
-                     ax.N == 1024 || ax.N == 2048;
-    return len && ((int64_t)ax.N * ax.inner + 512) * (int64_t)sizeof(cx<float>) < (int64_t(1) << 32);
-}
-
-int reg_threads_per_pencil(int N, int NC, int kind) { return N / reg_elems(N, NC, kind); }
 
 // Dynamic LDS beyond 64 KiB has to be enabled per kernel (up to the 160 KiB of a gfx950 CU).
 template <auto KERNEL>
@@ -492,52 +285,6 @@ int launch_reg_kind(int N, bool three, int chi_kind, const TileCfg& t, const Pas
             default: return fail(MMX_EINVAL, "internal: chi kind");
         }
     }
-}
-
-// Tile shape of a register-path pass: P pencils (a power of two) of TP threads each, at most 512 threads.
-int reg_tile_config(const AxisPlan& ax, int ncomp, bool fused, int64_t nbatch, int p_request, int kind, TileCfg& t) {
-    const int NC = fused ? ncomp : 1;
-    const int TP = reg_threads_per_pencil(ax.N, NC, kind);
-    const bool rows = ax.inner == 1;
-    t.nouter = fused ? ax.nouter : ax.nouter * nbatch;
-    const int64_t mult = fused ? nbatch : 1;
-    t.pad = 0;
-    t.big = 0;
-    int pmax = 512 / TP;
-    if (pmax < 1) return fail(MMX_EUNSUP, "internal: register FFT pencil too long");
-    // defaults from the tile sweep at 128^3 and 256^3: 8 rows of 3 components / 16 rows of 1; 32 columns for
-    // single-component strided passes; 8 columns of 3 components (16 of 1) for the Green pass
-    int want = p_request > 0 ? p_request : (rows ? (NC == 3 ? 8 : 16) : (fused ? (NC == 3 ? 8 : 16) : 32));
-    if (want > pmax) want = pmax;
-    // Small grids: smaller tiles give more CUs work, down to 4 rows per tile, and to 16 columns (128-byte row
-    // segments) for single-component strided passes or 4 for fused ones. A sweep of 128^2-512^2 and 64^3 grids
-    // measured smaller tiles slower (a 512^2 3-component FFT: 28 us with 1-row / 2-column tiles, 15 us with 8 / 16;
-    // a 256^2 scalar Born iteration: 22.4 us with 1 / 2, 19.3 us with 4 / 4; a 64^3 vector one: 46.6 us with 4-row
-    // tiles, 42.4 us with 8).
-    const int rows_min = 4, cols_min = fused ? 4 : 16;
-    if (rows) {
-        const int64_t total_rows = t.nouter * mult;
-        if (p_request <= 0) {
-            while (want > rows_min && total_rows / want < 256) want >>= 1;  // keep >= ~256 tiles when possible
-            while (want > 1 && want > t.nouter) want >>= 1;
-        }
-        t.P = pow2floor(want);
-        t.ntile_inner = 1;
-        t.tiles_per_problem = (t.nouter + t.P - 1) / t.P;
-    } else {
-        int P = pow2floor(want);
-        while (P > 1 && P > ax.inner) P >>= 1;
-        if (p_request <= 0)
-            while (P > cols_min && t.nouter * mult * ((ax.inner + P - 1) / P) < 512) P >>= 1;
-        t.P = P;
-        t.ntile_inner = (ax.inner + P - 1) / P;
-        t.tiles_per_problem = t.nouter * t.ntile_inner;
-    }
-    t.nth = t.P * TP;
-    const int E = ax.N / TP;
-    t.lds = (size_t)(ax.N + (size_t)t.P * padded_len(ax.N, E)) * sizeof(cx<float>);
-    t.nblocks = t.tiles_per_problem * mult;
-    return MMX_OK;
 }
 
 // Launch one pencil pass (see tile_config for `fused`).
@@ -1334,7 +1081,7 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
     for (int a = 0; a < 3; ++a) { p->s_lo[a] = 0; p->s_hi[a] = p->shape3[a]; }
     p->k1ofs = (int)p->shape3[0];
     p->k2ofs = (int)(p->shape3[0] + p->shape3[1]);
-    const std::vector<double> k = k_vectors(p);
+    const std::vector<double> k = k_vectors(p->shape3, p->step3, p->transposed);
     int rc = dtype == MMX_C64 ? upload_real<float>(k, &p->kvec) : upload_real<double>(k, &p->kvec);
     if (rc) { mmx_plan_destroy(p); return rc; }
     *out = p;
@@ -1380,7 +1127,7 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
             const int t = value ? 1 : 0;
             if (t != p->transposed) {
                 p->transposed = t;
-                const std::vector<double> k = k_vectors(p);
+                const std::vector<double> k = k_vectors(p->shape3, p->step3, p->transposed);
                 const int rc = p->dtype == MMX_C64 ? reupload_real<float>(k, p->kvec)
                                                    : reupload_real<double>(k, p->kvec);
                 if (rc) return rc;

@@ -7,9 +7,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mmx_plan.hpp"
+
 namespace mmx {
 
-constexpr int kMaxRadices = 12;
 
 template <typename T>
 struct alignas(2 * sizeof(T)) cx {
@@ -37,40 +38,6 @@ template <typename T> __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
     T den = b.r * b.r + b.i * b.i;
     T inv = T(1) / den;
     return {(a.r * b.r + a.i * b.i) * inv, (a.i * b.r - a.r * b.i) * inv};
-}
-
-// ----------------------------------------------------------------------------------------------
-// Division by a launch-constant divisor without the integer-divide sequence: a float reciprocal gives the
-// quotient to within one, the remainder test corrects it. Exact for 0 <= x < 2^22.
-// ----------------------------------------------------------------------------------------------
-struct FDiv {
-    int d;
-    float inv;
-};
-
-__host__ inline FDiv make_fdiv(int d) { return FDiv{d, 1.0f / float(d)}; }
-__device__ __forceinline__ FDiv make_fdiv_dev(int d) { return FDiv{d, 1.0f / float(d)}; }
-
-__device__ __forceinline__ int fdiv(int x, const FDiv& f) {
-    int q = __float2int_rz(__int2float_rn(x) * f.inv);
-    const int r = x - q * f.d;
-    q += (r >= f.d) - (r < 0);
-    return q;
-}
-
-// 64-bit variant through a double reciprocal (exact for 0 <= x < 2^50).
-struct FDiv64 {
-    int64_t d;
-    double inv;
-};
-
-__host__ inline FDiv64 make_fdiv64(int64_t d) { return FDiv64{d, 1.0 / double(d)}; }
-
-__device__ __forceinline__ int64_t fdiv64(int64_t x, const FDiv64& f) {
-    int64_t q = (int64_t)(double(x) * f.inv);
-    const int64_t r = x - q * f.d;
-    q += (r >= f.d) - (r < 0);
-    return q;
 }
 
 // ----------------------------------------------------------------------------------------------

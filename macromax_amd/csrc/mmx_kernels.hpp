@@ -259,17 +259,6 @@ __device__ __forceinline__ void write_partials(double dd, double ee, const PassA
     }
 }
 
-// Pass kinds: which prologue / transform / epilogue a pencil pass runs.
-enum PassKind : int {
-    kFwd = 0,     // load, forward FFT, store
-    kInv = 1,     // load, inverse FFT, store * scale
-    kProFwd = 2,  // Born prologue (chi E + s), forward FFT, store
-    kGreen = 3,   // load, forward FFT, Green's operator, inverse FFT, store
-    kInvEpi = 4,  // load, inverse FFT, Born epilogue (update + norms)
-    kAll = 5,     // Born prologue, forward, Green, inverse, Born epilogue (single-axis grids)
-    kEpiPro = 6,  // load, inverse FFT, Born epilogue, then the next iteration's prologue and forward FFT, store
-    kGreenEpiPro = 7,  // single-axis grids: load (k-space), Green, inverse, epilogue, prologue, forward, store
-};
 
 template <int KIND>
 struct PassFlags {
@@ -314,7 +303,7 @@ __device__ void plain_transform(cx<T>* tile, const cx<T>* tw, const TileGeom& g,
         return;
     }
     const int NT = a.NT;
-    const FDiv fNT = make_fdiv_dev(NT);
+    const FDiv fNT = make_fdiv(NT);
     for (int idx = tid; idx < P * NT; idx += nth) {
         int p, j;
         if (rows) { p = fdiv(idx, fNT); j = idx - p * NT; }

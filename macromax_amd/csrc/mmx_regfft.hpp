@@ -112,9 +112,7 @@ __device__ __forceinline__ void load_twiddles(cx<T>* tw, const cx<T>* g, int tid
 // strides of the exchange. LDS has 64 four-byte banks; an 8-byte read serves 32 lanes per cycle and an 8-byte write
 // 16. With E = 8 (three radix-8/4 stages) a slot every 8 entries makes the strided-tile exchange and the rows-tile
 // writes conflict-free (rows-tile reads then pair up 3 of 32 lanes); with E = 16 a slot every 16 is conflict-free.
-constexpr int pad_shift(int E) { return E == 8 ? 3 : 4; }  // 12 and 24 points per thread: a slot every 16
 template <int E> __device__ __forceinline__ int padpos(int pos) { return pos + (pos >> pad_shift(E)); }
-constexpr int padded_len(int N, int E) { return N + (N >> pad_shift(E)); }
 
 
 template <int N, int E> constexpr bool reg_schedule_ok() {
