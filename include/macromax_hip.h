@@ -73,6 +73,10 @@ int mmx_plan_destroy(mmx_plan_t plan);
  *   MMX_PARAM_SCRATCH_GEN read-only (mmx_plan_get_param): how many times the plan's device scratch buffer (FP64
  *                        partial sums) has been allocated. Launches recorded in a graph hold that buffer's address,
  *                        so a caller replaying a graph re-records it when this number has changed since capture.
+ *   MMX_PARAM_RADIX_ORDER radix order of the LDS-staged transforms (every axis the register kernels do not take):
+ *                        0 (default) radix 8 first, then 4, 2, 3, 5, 7; 1 radix 4 first (4, 2, 8, 3, 5, 7);
+ *                        2 the odd radices first (7, 5, 3, 8, 4, 2). Stages run in the order their radices are
+ *                        taken. Values above 2 are rejected (MMX_EINVAL).
  */
 #define MMX_PARAM_G_AXIS 1
 #define MMX_PARAM_P_ROWS 2
@@ -84,6 +88,7 @@ int mmx_plan_destroy(mmx_plan_t plan);
 #define MMX_PARAM_REG_FFT 8
 #define MMX_PARAM_TRANSPOSED 9
 #define MMX_PARAM_SCRATCH_GEN 10
+#define MMX_PARAM_RADIX_ORDER 11
 int mmx_plan_set_param(mmx_plan_t plan, int param, int64_t value);
 /*
  * Support of the sources s of the Born calls that follow on this plan: s is zero outside the box

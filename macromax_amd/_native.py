@@ -33,7 +33,7 @@ TRACK = {'prev': 0, 'field_norm': 1, 'residue': 2, 'iter': 3, 'alpha_r': 4, 'alp
 TRACK_LEN = 12
 
 PARAMS = {'g_axis': 1, 'p_rows': 2, 'p_strided': 3, 'p_green': 4, 'xcd_remap': 5, 'lds_budget': 6,
-          'big_elems': 7, 'reg_fft': 8, 'transposed': 9, 'scratch_gen': 10}
+          'big_elems': 7, 'reg_fft': 8, 'transposed': 9, 'scratch_gen': 10, 'radix_order': 11}
 
 
 class NativeError(RuntimeError):

@@ -1,6 +1,7 @@
 """Autotuner for the pencil-pass kernels: picks the tile shapes, the axis of the fused Green pass, the kernel family
-(register-resident radix-16/8 or LDS-staged radix-8/4/2) and the XCD block order for one grid shape, element type,
-field and chi kind and batch size, by timing the fused iteration (``mmx_born_iterate``) on the device.
+(register-resident radix-16/8 or LDS-staged radix-8/4/2), the radix order of the LDS-staged transforms and the XCD
+block order for one grid shape, element type, field and chi kind and batch size, by timing the fused iteration
+(``mmx_born_iterate``) on the device.
 
 No reference counterpart: the reference calls FFT libraries (pyFFTW plans, ``backend/numpy.py:46-89``;
 PyTorch's FFT module). The search is coordinate descent over ``mmx_plan_set_param`` parameters (each parameter in turn, the
@@ -25,6 +26,7 @@ __all__ = ['tune_plan', 'tune_solution', 'apply', 'cache_key', 'PARAMETERS']
 PARAMETERS = {
     'g_axis': (1, 0),
     'reg_fft': (1, 0),
+    'radix_order': (0, 1, 2),
     'p_rows': (0, 4, 8, 16, 32),
     'p_strided': (0, 8, 16, 32, 64),
     'p_green': (0, 4, 8, 16),
@@ -75,6 +77,13 @@ def apply(plan, params: dict):
         plan.set_param(name, int(value))
 
 
+def _lds_axes(plan) -> bool:
+    """Whether some axis of the plan runs on the LDS-staged kernels whatever reg_fft says (complex128, or an axis
+    length without register kernels)."""
+    from . import _native
+    return plan.dtype != torch.complex64 or any(not _native.register_fft_length(n) for n in plan.shape if n > 1)
+
+
 def _candidates(plan):
     out = {}
     from . import _native
@@ -84,6 +93,8 @@ def _candidates(plan):
             continue
         if name == 'reg_fft' and (not reg or plan.dtype != torch.complex64):
             continue
+        if name == 'radix_order' and not _lds_axes(plan) and 'reg_fft' not in out:
+            continue  # every axis runs on the register kernels, whose radices are fixed
         if name == 'p_strided' and sum(1 for n in plan.shape if n > 1) < 3:
             continue
         out[name] = values
@@ -159,6 +170,8 @@ def tune_plan(plan, ncomp: int, chi_kind: int, nbatch: int = 1, chi_shared: bool
     best = {name: values[0] for name, values in cands.items()}
     best_ms = measure(best)
     for name, values in cands.items():
+        if name == 'radix_order' and best.get('reg_fft', 1) and not _lds_axes(plan):
+            continue  # the register kernels won: no axis uses the radix order
         for v in values:
             if v == best[name]:
                 continue

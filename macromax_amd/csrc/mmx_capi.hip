@@ -61,6 +61,7 @@ struct mmx_plan_s {
     int xcd_remap;
     int big_elems;   // bit k set: pass kind k uses the larger per-thread element budget
     int reg_fft;     // 1: power-of-two axes use the register-resident FFT kernels
+    int radix_order; // radix order of the LDS-staged transforms (factorize)
     size_t lds_budget;
 };
 
@@ -160,7 +161,7 @@ int init_axis(mmx_plan_s* p, AxisPlan& ax, int64_t n, int64_t inner, int64_t nou
     } else {
         return fail(MMX_EUNSUP, "internal: axis of " + std::to_string(n) + " points in one pass");
     }
-    if (!factorize(ax.NT, ax.rad, ax.nrad)) return fail(MMX_EUNSUP, "internal: transform length radices");
+    if (!factorize(ax.NT, ax.rad, ax.nrad, p->radix_order)) return fail(MMX_EUNSUP, "internal: transform length radices");
     int rc = p->dtype == MMX_C64 ? make_twiddles<float>(ax.NT, &ax.tw) : make_twiddles<double>(ax.NT, &ax.tw);
     if (!rc && ax.blue) rc = p->dtype == MMX_C64 ? make_bluestein<float>(ax) : make_bluestein<double>(ax);
     return rc;
@@ -1039,6 +1040,7 @@ int mmx_plan_create(mmx_plan_t* out, int ndim, const int64_t* shape, const doubl
     // XCD (their 64-byte row segments share L2 lines); the plain passes do not
     p->xcd_remap = (1 << kGreen) | (1 << kInvEpi) | (1 << kEpiPro) | (1 << kGreenEpiPro);
     p->reg_fft = 1;
+    p->radix_order = 0;
     p->lds_budget = kLdsBudget;
     p->fused_ok[0] = p->fused_ok[1] = -1;
     // non-trivial axes, outermost first
@@ -1134,6 +1136,20 @@ int mmx_plan_set_param(mmx_plan_t p, int param, int64_t value) {
             }
             break;
         }
+        case MMX_PARAM_RADIX_ORDER: {
+            if (value >= kRadixOrders) return fail(MMX_EINVAL, "radix order must be 0, 1 or 2");
+            // every transform length is 7-smooth, so each order factorises it; only the stage list changes
+            // (a four-step axis is transformed by the passes of its two sub-axes)
+            for (int q = 0; q < p->naxes; ++q) {
+                AxisPlan* axs[2] = {&p->sub[q][0], &p->sub[q][1]};
+                if (!p->ax[q].split) axs[0] = &p->ax[q];
+                for (int i = 0; i < (p->ax[q].split ? 2 : 1); ++i)
+                    if (!factorize(axs[i]->NT, axs[i]->rad, axs[i]->nrad, (int)value))
+                        return fail(MMX_EUNSUP, "internal: transform length radices");
+            }
+            p->radix_order = (int)value;
+            break;
+        }
         case MMX_PARAM_SCRATCH_GEN: return fail(MMX_EINVAL, "MMX_PARAM_SCRATCH_GEN is read-only");
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
@@ -1154,6 +1170,7 @@ int mmx_plan_get_param(mmx_plan_t p, int param, int64_t* value) {
         case MMX_PARAM_REG_FFT: *value = p->reg_fft; break;
         case MMX_PARAM_TRANSPOSED: *value = p->transposed; break;
         case MMX_PARAM_SCRATCH_GEN: *value = p->scratch_gen; break;
+        case MMX_PARAM_RADIX_ORDER: *value = p->radix_order; break;
         default: return fail(MMX_EINVAL, "unknown parameter");
     }
     return MMX_OK;

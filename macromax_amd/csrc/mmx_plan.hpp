@@ -32,6 +32,7 @@ enum PassKind : int {
 constexpr int pad_shift(int E) { return E == 8 ? 3 : 4; }  // 12 and 24 points per thread: a slot every 16
 constexpr int padded_len(int N, int E) { return N + (N >> pad_shift(E)); }
 
+
 // Division by a launch-constant divisor without the integer-divide sequence: a float reciprocal gives the quotient to
 // within one, the remainder test corrects it. Exact for 0 <= x < 2^22 and 1 <= d < 2^22 (x and d are exact in
 // float, the estimate x * (1/d) carries a relative error below 2^-23, so |estimate - x/d| < 1/2). constexpr makes the
@@ -99,11 +100,17 @@ struct AxisPlan {
 };
 
 
-inline bool factorize(int n, int* rad, int& nrad) {
-    static const int pref[] = {8, 4, 2, 3, 5, 7};
+// Radix orders of the LDS-staged Stockham transforms (plan parameter MMX_PARAM_RADIX_ORDER): each takes the radices
+// greedily in its order of preference, and its stages run in that order. 0: radix 8 first (fewest stages);
+// 1: radix 4 first (more stages, fewer registers per butterfly, no idle lanes at four points per thread);
+// 2: the odd radices first.
+constexpr int kRadixOrders = 3;
+inline bool factorize(int n, int* rad, int& nrad, int order = 0) {
+    static const int prefs[kRadixOrders][6] = {{8, 4, 2, 3, 5, 7}, {4, 2, 8, 3, 5, 7}, {7, 5, 3, 8, 4, 2}};
     nrad = 0;
+    if (order < 0 || order >= kRadixOrders) return false;
     int m = n;
-    for (int r : pref) {
+    for (int r : prefs[order]) {
         while (m % r == 0 && m > 1) {
             if (nrad >= kMaxRadices) return false;
             rad[nrad++] = r;

@@ -57,20 +57,44 @@ static void test_smooth_and_factorize() {
     for (int n = 1; n < lim; ++n) {
         const bool smooth = lpf[n] <= 7;
         CHECK(smooth7(n) == smooth, "n=%d", n);
-        int rad[kMaxRadices], nrad = -1;
-        const bool ok = factorize(n, rad, nrad);
-        if (n <= kMaxN) {
-            CHECK(ok == smooth, "factorize(%d)", n);
-            if (ok) {
-                int64_t prod = 1;
-                for (int i = 0; i < nrad; ++i) {
-                    CHECK(rad[i] == 2 || rad[i] == 3 || rad[i] == 4 || rad[i] == 5 || rad[i] == 7 || rad[i] == 8,
-                          "n=%d radix %d", n, rad[i]);
-                    prod *= rad[i];
-                }
-                CHECK(prod == n, "n=%d product %lld", n, (long long)prod);
-                CHECK(nrad <= kMaxRadices, "n=%d nrad=%d", n, nrad);
+        for (int order = 0; order < kRadixOrders; ++order) {
+            int rad[kMaxRadices], nrad = -1;
+            const bool ok = factorize(n, rad, nrad, order);
+            if (n > kMaxN) continue;
+            CHECK(ok == smooth, "factorize(%d, order %d)", n, order);
+            if (!ok) continue;
+            int64_t prod = 1;
+            int n8 = 0, n4 = 0, n2 = 0;
+            for (int i = 0; i < nrad; ++i) {
+                CHECK(rad[i] == 2 || rad[i] == 3 || rad[i] == 4 || rad[i] == 5 || rad[i] == 7 || rad[i] == 8,
+                      "n=%d radix %d", n, rad[i]);
+                prod *= rad[i];
+                n8 += rad[i] == 8;
+                n4 += rad[i] == 4;
+                n2 += rad[i] == 2;
             }
+            CHECK(prod == n, "n=%d order %d product %lld", n, order, (long long)prod);
+            CHECK(nrad <= kMaxRadices, "n=%d nrad=%d", n, nrad);
+            // the power-of-two part 2^e: order 1 takes radix 4 while it divides, so no 8 and at most one 2;
+            // the others take 8 first, then at most one 4 or one 2
+            int e = 0;
+            for (int m = n; m % 2 == 0; m /= 2) ++e;
+            if (order == 1) CHECK(n8 == 0 && n4 == e / 2 && n2 == e % 2, "n=%d order 1", n);
+            else CHECK(n8 == e / 3 && n4 == (e % 3 == 2) && n2 == (e % 3 == 1), "n=%d order %d", n, order);
+            // order 2 runs every odd radix before the power-of-two ones, the others after them
+            bool seen_even = false, seen_odd = false, odd_after_even = false, even_after_odd = false;
+            for (int i = 0; i < nrad; ++i) {
+                const bool odd = rad[i] % 2 == 1;
+                if (odd && seen_even) odd_after_even = true;
+                if (!odd && seen_odd) even_after_odd = true;
+                seen_even |= !odd;
+                seen_odd |= odd;
+            }
+            CHECK(order == 2 ? !odd_after_even : !even_after_odd, "n=%d order %d stage order", n, order);
+        }
+        {
+            int rad[kMaxRadices], nrad = 0;
+            CHECK(!factorize(n, rad, nrad, kRadixOrders), "order %d accepted", kRadixOrders);
         }
     }
     CHECK(!smooth7(0) && !smooth7(-4), "non-positive lengths are not 7-smooth");

@@ -58,6 +58,33 @@ def test_fft_matches_numpy(shape, dtype, device):
     assert rel_l2(out.cpu().numpy(), x) < tol
 
 
+RADIX_SHAPES = [(105,), (56, 40), (5, 10, 20), (64, 96), (96, 24, 40), (13, 17, 11), (4097,)]
+
+
+@pytest.mark.parametrize('shape', RADIX_SHAPES, ids=lambda s: 'x'.join(map(str, s)))
+@pytest.mark.parametrize('dtype', ['c64', 'c128'])
+@pytest.mark.parametrize('order', [1, 2])
+def test_fft_radix_orders_match_numpy(shape, dtype, order, device):
+    """The LDS-staged transforms with the other radix orders (MMX_PARAM_RADIX_ORDER; the register kernels off)."""
+    import torch
+    print(f'fft radix order {order} {shape} {dtype}', flush=True)
+    rng = np.random.default_rng(sum(shape) + order)
+    x = rng.standard_normal((2, *shape)) + 1j * rng.standard_normal((2, *shape))
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    tol = 2e-6 if dtype == 'c64' else 1e-12
+    plan = _plan(shape, [1.0] * len(shape), tdt, device)
+    plan.set_param('reg_fft', 0)
+    plan.set_param('radix_order', order)
+    assert plan.get_param('radix_order') == order
+    xt = torch.as_tensor(x, dtype=tdt, device=device).contiguous()
+    out = torch.empty_like(xt)
+    plan.fft(xt, out, -1)
+    axes = tuple(range(1, 1 + len(shape)))
+    assert rel_l2(out.cpu().numpy(), np.fft.fftn(x, axes=axes)) < tol
+    plan.fft(out, out, +1)
+    assert rel_l2(out.cpu().numpy(), x) < tol
+
+
 def _born_case(shape, C, kind, nbatch=1, seed=0, chi_shared=True):
     rng = np.random.default_rng(seed)
     V = int(np.prod(shape))
@@ -169,6 +196,36 @@ def test_born_step_matches_oracle(shape, C, kind, dtype, device):
     assert abs(ee - np.sum(np.abs(en) ** 2)) / np.sum(np.abs(en) ** 2) < 2 * tol
     # the input field is untouched
     assert np.array_equal(Et.cpu().numpy(), E[0].astype(Et.cpu().numpy().dtype))
+
+
+@pytest.mark.parametrize('shape,C,kind,dtype', [((24, 40, 56), 3, 4, 'c64'), ((96, 80), 1, 1, 'c64'),
+                                                ((16, 20, 24), 3, 2, 'c128'), ((13, 17, 11), 3, 3, 'c64')],
+                         ids=lambda v: str(v))
+@pytest.mark.parametrize('order', [0, 1, 2])
+def test_born_step_radix_orders_match_oracle(shape, C, kind, dtype, order, device):
+    """One Born step on the LDS-staged kernels for every radix order, against the oracle."""
+    import torch
+    print(f'born radix order {order} {shape} C={C} chi_kind={kind} {dtype}', flush=True)
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, seed=order + 5)
+    step = [0.41 + 0.03 * i for i in range(len(shape))]
+    k = ob.k_vectors(shape, step)
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    rdt = torch.float32 if dtype == 'c64' else torch.float64
+    tol = 1e-5 if dtype == 'c64' else 1e-11
+    plan = _plan(shape, step, tdt, device)
+    plan.set_param('reg_fft', 0)
+    plan.set_param('radix_order', order)
+    Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
+    Eo = torch.zeros_like(Et)
+    st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64 if tdt == torch.complex64 else np.complex128),
+                         device=device).contiguous()
+    coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
+    norms = torch.zeros(2, dtype=torch.float64, device=device)
+    plan.born_step(Et, Eo, st, ct, C, kind, coef, 1, 0, 0, norms)
+    torch.cuda.synchronize()
+    d_ref = _oracle_update(E[0], s[0], chi[0], alphas[0], a[0], c[0], k, C == 3)
+    assert rel_l2(Eo.cpu().numpy() - E[0], d_ref) < tol
 
 
 @pytest.mark.parametrize('shape,C,kind,shared', [((64, 48), 1, 1, True), ((12, 16, 20), 3, 2, True),
