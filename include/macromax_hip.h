@@ -178,7 +178,10 @@ int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const 
  * field ([1,1,grid]), 2 for a 3x3 field ([3,3,grid] row-major); chi_vstride[i] 1 for a field, 0 for a constant (one
  * value or one 3x3 matrix). E, Eout, s: [3,1,grid]; work: 12 * grid complex values of scratch (distinct from E and
  * Eout); norms (device, 2 doubles): {sum |d|^2, sum |Eout|^2} in FP64. The caller applies the accept rule
- * (solver.py:1078-1096) as for mmx_born_step.
+ * (solver.py:1078-1096) as for mmx_born_step. Complex64 plans whose first and last axes have 64, 128, 256, 512 or
+ * 1024 points (2D or 3D, register kernels on) run fused pencil passes: the k-space products inside the passes along
+ * the first axis, the voxel-wise products inside those along the last axis (eight kernels per 2D iteration); other
+ * plans run the transforms and five voxel-wise kernels separately. Both compute the same iteration.
  */
 int mmx_magnetic_iterate(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* const* chi,
                          const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,

@@ -15,6 +15,7 @@
 #include "mmx_magnetic.hpp"
 #include "mmx_pointwise.hpp"
 #include "mmx_regkernels.hpp"
+#include "mmx_magfused.hpp"
 #include "mmx_setup.hpp"
 
 using namespace mmx;
@@ -864,6 +865,143 @@ int batch_step_impl(mmx_plan_s* p, void* EA, void* EB, void* W, const void* s, c
 }
 
 // One magnetic Born iteration (see mmx_magnetic.hpp for the sequence).
+// ---- fused magnetic iteration (mmx_magfused.hpp): complex64, first and last axes of 64..1024 points (powers of two)
+#ifndef MMX_MAG_FUSED
+#define MMX_MAG_FUSED 1
+#endif
+inline bool mag_fused_length(const AxisPlan& ax) {
+    const int n = ax.N;
+    return (n == 64 || n == 128 || n == 256 || n == 512 || n == 1024) && reg_supported(ax);
+}
+
+template <int N, int OP>
+void launch_mag_mid(const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a, const cx<float>* src,
+                    cx<float>* dst, hipStream_t st) {
+    constexpr int E = reg_elems(N, 3, kGreen);
+    constexpr int NC = OP == 1 ? 6 : 3;
+    allow_lds<&mag_mid_kernel<float, N, E, NC, OP>>(t.lds);
+    hipLaunchKernelGGL((mag_mid_kernel<float, N, E, NC, OP>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
+                       src, dst);
+}
+
+template <int N, int STAGE>
+void launch_mag_pw(const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a, const cx<float>* src,
+                   cx<float>* dst, hipStream_t st) {
+    constexpr int E = reg_elems(N, 3, kEpiPro);
+    allow_lds<&mag_pw_kernel<float, N, E, STAGE>>(t.lds);
+    hipLaunchKernelGGL((mag_pw_kernel<float, N, E, STAGE>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
+                       src, dst);
+}
+
+// step 0..2: mid pass with k-space product OP = step; 3..5: pw pass STAGE = step - 2
+template <int N>
+void launch_mag_step(int step, const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a,
+                     const cx<float>* src, cx<float>* dst, hipStream_t st) {
+    switch (step) {
+        case 0: launch_mag_mid<N, 0>(t, g, a, src, dst, st); break;
+        case 1: launch_mag_mid<N, 1>(t, g, a, src, dst, st); break;
+        case 2: launch_mag_mid<N, 2>(t, g, a, src, dst, st); break;
+        case 3: launch_mag_pw<N, 1>(t, g, a, src, dst, st); break;
+        case 4: launch_mag_pw<N, 2>(t, g, a, src, dst, st); break;
+        default: launch_mag_pw<N, 3>(t, g, a, src, dst, st); break;
+    }
+}
+
+// Geometry and tiles of a fused magnetic pass along `ax` (mid passes: the first axis, strided tiles of three
+// components as the Green pass; pw passes: the last axis, row tiles as the update pass).
+int mag_geometry(const mmx_plan_s* p, const AxisPlan& ax, bool mid, PassArgs<float>& g, TileCfg& t) {
+    g = blank_args<float>();
+    fill_geometry<float>(p, ax, g);
+    const int kind = mid ? kGreen : kEpiPro;
+    const int rc = reg_tile_config(ax, 3, true, 1, mid ? p->p_green : p->p_rows, kind, t);
+    if (rc) return rc;
+    g.nouter = t.nouter;
+    g.P = t.P;
+    g.ntile_inner = (int)t.ntile_inner;
+    g.tiles_per_problem = t.tiles_per_problem;
+    g.fntile = make_fdiv64(t.ntile_inner);
+    g.ftpp = make_fdiv64(t.tiles_per_problem);
+    g.xcd_remap = (p->xcd_remap >> kind) & 1;
+    g.scale = mid ? 1.0f : float(1.0 / double(p->V));
+    if (t.nblocks <= 0 || t.nblocks > 0x7fffffffLL) return fail(MMX_EINVAL, "internal: magnetic pass grid");
+    return MMX_OK;
+}
+
+int mag_launch(const mmx_plan_s* p, int step, const MagArgs<float>& a, const cx<float>* src, cx<float>* dst,
+               hipStream_t st) {
+    const bool mid = step < 3;
+    const AxisPlan& ax = mid ? p->ax[0] : p->ax[p->naxes - 1];
+    PassArgs<float> g;
+    TileCfg t;
+    int rc = mag_geometry(p, ax, mid, g, t);
+    if (rc) return rc;
+    switch (ax.N) {
+        case 64: launch_mag_step<64>(step, t, g, a, src, dst, st); break;
+        case 128: launch_mag_step<128>(step, t, g, a, src, dst, st); break;
+        case 256: launch_mag_step<256>(step, t, g, a, src, dst, st); break;
+        case 512: launch_mag_step<512>(step, t, g, a, src, dst, st); break;
+        case 1024: launch_mag_step<1024>(step, t, g, a, src, dst, st); break;
+        default: return fail(MMX_EINVAL, "internal: fused magnetic pass length");
+    }
+    MMX_HIP_CHECK(hipGetLastError());
+    return MMX_OK;
+}
+
+// One plain pass along the middle axis of a 3D grid (nfields fields, unscaled).
+int mag_middle_pass(const mmx_plan_s* p, const cx<float>* src, cx<float>* dst, int64_t nfields, int direction,
+                    hipStream_t st) {
+    PassArgs<float> a = blank_args<float>();
+    fill_geometry<float>(p, p->ax[1], a);
+    a.src = src;
+    a.dst = dst;
+    a.dir = direction;
+    return launch_pass<float>(p, p->ax[1], a, nfields, false, direction < 0 ? kFwd : kInv, st);
+}
+
+bool mag_fused_ok(const mmx_plan_s* p) {
+    if (!MMX_MAG_FUSED || p->dtype != MMX_C64 || !p->reg_fft || p->naxes < 2 || p->naxes > 3) return false;
+    if (!mag_fused_length(p->ax[0]) || !mag_fused_length(p->ax[p->naxes - 1])) return false;
+    return p->naxes == 2 || !p->ax[1].split;
+}
+
+// The iteration with the fused passes (see mmx_magfused.hpp for the schedule).
+int magnetic_fused(mmx_plan_s* p, MagArgs<float>& a, double* norms, hipStream_t st) {
+    const int64_t V = p->V;
+    const bool three = p->naxes == 3;
+    PassArgs<float> gf;
+    TileCfg tf;
+    int rc = mag_geometry(p, p->ax[p->naxes - 1], false, gf, tf);
+    if (rc) return rc;
+    rc = ensure_scratch(p, (size_t)(2 * tf.nblocks) * sizeof(double));
+    if (rc) return rc;
+    a.partials = reinterpret_cast<double*>(p->scratch);
+    cx<float>* w1 = a.w1;
+    cx<float>* w2 = a.w2;
+    cx<float>* w3 = a.w3;
+    // forward transform of E along the last axis (and the middle one): plain passes, three fields
+    {
+        PassArgs<float> b = blank_args<float>();
+        fill_geometry<float>(p, p->ax[p->naxes - 1], b);
+        b.src = a.E;
+        b.dst = w2;
+        b.dir = -1;
+        if ((rc = launch_pass<float>(p, p->ax[p->naxes - 1], b, 3, false, kFwd, st))) return rc;
+    }
+    if (three && (rc = mag_middle_pass(p, w2, w2, 3, -1, st))) return rc;
+    if ((rc = mag_launch(p, 0, a, w2, w2, st))) return rc;       // E^ -> w1, i k x E^
+    if (three && (rc = mag_middle_pass(p, w2, w2, 3, +1, st))) return rc;
+    if ((rc = mag_launch(p, 3, a, w2, w3, st))) return rc;       // curl E; [Z1; Y1], forward along the last axis
+    if (three && (rc = mag_middle_pass(p, w3, w3, 6, -1, st))) return rc;
+    if ((rc = mag_launch(p, 1, a, w3, w3, st))) return rc;       // [B^; i k x B^]
+    if (three && (rc = mag_middle_pass(p, w3, w3, 6, +1, st))) return rc;
+    if ((rc = mag_launch(p, 4, a, w3, w2, st))) return rc;       // [B; curl B]; Z2 -> w1; Y2 forward
+    if (three && (rc = mag_middle_pass(p, w2, w2, 3, -1, st))) return rc;
+    if ((rc = mag_launch(p, 2, a, w2, w2, st))) return rc;       // i k x Y2^
+    if (three && (rc = mag_middle_pass(p, w2, w2, 3, +1, st))) return rc;
+    if ((rc = mag_launch(p, 5, a, w2, w1, st))) return rc;       // curl Y2; d, E', partials
+    return reduce_norms(a.partials, tf.nblocks, 1, norms, st);
+}
+
 template <typename T>
 int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* const* chi, const int* kinds,
                   const int64_t* vstride, const double* alpha, void* work, double* norms, hipStream_t st) {
@@ -898,6 +1036,9 @@ int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const
     a.nd = p->ndim;
     a.alpha_r = T(alpha[0]);
     a.alpha_i = T(alpha[1]);
+    if constexpr (sizeof(T) == 4) {
+        if (mag_fused_ok(p)) return magnetic_fused(p, a, norms, st);
+    }
     const int nblk = pointwise_blocks(V);
     int rc = ensure_scratch(p, (size_t)(2 * nblk) * sizeof(double));
     if (rc) return rc;

@@ -1,0 +1,288 @@
+// Fused pencil passes of the magnetic iteration (complex64, register-resident FFT lengths 64-1024 on the first and
+// last grid axes). The iteration of mmx_magnetic.hpp,
+//
+//     E^ = F E, curl E;  [Z1; Y1] = [chi_EE E + chi_HE curl E + s; chi_EH E + chi_HH curl E];
+//     B^ = G (Z1^ + i k x Y1^) - E^;  [Z2; Y2] = [chi_EE B + chi_HE curl B; chi_EH B + chi_HH curl B];
+//     d = Z2 + curl Y2,  E' = E + d,
+//
+// is evaluated with the k-space products inside the pass along the first axis ("mid" passes: forward FFT, product,
+// inverse FFT) and the real-space products inside the pass along the last, contiguous axis ("pw" passes: inverse FFT
+// of the transform that ends there, the voxel-wise products, forward FFT of the transform that starts there). With
+// plain passes along the middle axis of 3D grids in between, one iteration is
+//
+//     fwd C (E) | [B fwd] | mid 1: fwd A, E^ -> w1, i k x, inv A | [B inv] | pw 1: inv C, [Z1; Y1], fwd C |
+//     [B fwd] | mid 2: fwd A, B^, i k x B^, inv A | [B inv] | pw 2: inv C, Z2 -> w1, Y2, fwd C | [B fwd] |
+//     mid 3: fwd A, i k x, inv A | [B inv] | final: inv C, d, E', norms
+//
+// (A the first axis, C the last), eight kernels on a 2D grid instead of nineteen. The inverse transforms' 1/N is
+// applied by the pw passes.
+#pragma once
+#include "mmx_magnetic.hpp"
+#include "mmx_regkernels.hpp"
+
+namespace mmx {
+
+// Pencil geometry of a register-path tile (as in reg_pass_kernel): pencil p, point t + m TP of it at byte offset
+// toff + m mstep from the tile's first element tb; `valid` false for pencils past the edge of the last tile.
+struct MagTile {
+    int64_t tb;
+    int64_t row, col;  // outer index (rows tiles: the row; strided: the block before the axis) and column
+    uint32_t toff, mstep;
+    bool valid;
+};
+
+template <typename T, int N, int E>
+__device__ __forceinline__ MagTile mag_tile(const PassArgs<T>& g, int p, int t) {
+    constexpr int TP = N / E;
+    constexpr uint32_t ES = sizeof(cx<T>);
+    int64_t blk = blockIdx.x;
+    if (g.xcd_remap) {
+        const int64_t n = gridDim.x, q = n >> 3, rem = n & 7, r = blk & 7;
+        blk = r * q + (r < rem ? r : rem) + (blk >> 3);
+    }
+    MagTile m;
+    if (g.inner == 1) {
+        const int64_t r0 = blk * g.P;
+        m.valid = r0 + p < g.nouter;
+        m.row = r0 + p;
+        m.col = 0;
+        m.tb = r0 * N;
+        m.toff = (uint32_t)(p * N + t) * ES;
+        m.mstep = (uint32_t)TP * ES;
+    } else {
+        const int64_t o = fdiv64(blk, g.fntile);
+        const int64_t ib = (blk - o * g.ntile_inner) * g.P;
+        m.valid = ib + p < g.inner;
+        m.row = o;
+        m.col = ib + p;
+        m.tb = o * (int64_t)N * g.inner + ib;
+        m.toff = ((uint32_t)p + (uint32_t)t * (uint32_t)g.inner) * ES;
+        m.mstep = (uint32_t)TP * (uint32_t)g.inner * ES;
+    }
+    return m;
+}
+
+template <typename T, int N, int E, int NC>
+__device__ __forceinline__ void mag_load(cx<T> (&v)[NC][E], const cx<T>* base, const MagTile& tl, int64_t V) {
+    const cx<T> zero{T(0), T(0)};
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c][m] = zero;
+    if (!tl.valid) return;
+    const cx<T>* b = base + tl.tb;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c][m] = ldg<true>(b + c * V, tl.toff + m * tl.mstep);
+}
+
+template <typename T, int N, int E, int NC>
+__device__ __forceinline__ void mag_store(const cx<T> (&v)[NC][E], cx<T>* base, const MagTile& tl, int64_t V) {
+    if (!tl.valid) return;
+    cx<T>* b = base + tl.tb;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+        for (int c = 0; c < NC; ++c) stg<true>(b + c * V, tl.toff + m * tl.mstep, v[c][m]);
+}
+
+// Mid passes along the first grid axis (g.axis_orig == 0): forward FFT of NC components of `src`, a k-space product,
+// inverse FFT, result to `dst` (unscaled).
+//   OP 0 (NC = 3): E^ to w1; i k x E^
+//   OP 1 (NC = 6): [Z1^; Y1^] -> B^ = G (Z1^ + i k x Y1^) - E^ (E^ from w1); [B^; i k x B^]
+//   OP 2 (NC = 3): i k x
+template <typename T, int N, int E, int NC, int OP>
+__global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
+    constexpr int TP = N / E;
+    constexpr int LTP = ilog2(TP);
+    extern __shared__ unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* xbuf = tw + N;
+    const int tid = threadIdx.x;
+    const bool rows = g.inner == 1;
+    const int P = g.P;
+    const int LP = 31 - __clz(P);
+    int p, t, ps;
+    cx<T>* xb;
+    if (rows) { t = tid & (TP - 1); p = tid >> LTP; xb = xbuf + p * padded_len(N, E); ps = 1; }
+    else { p = tid & (P - 1); t = tid >> LP; xb = xbuf + p; ps = P; }
+    const MagTile tl = mag_tile<T, N, E>(g, p, t);
+    const int64_t V = a.V;
+    load_twiddles<T, N, E>(tw, g.tw, tid, blockDim.x);
+    cx<T> v[NC][E];
+    mag_load<T, N, E, NC>(v, src, tl, V);
+    // k off the pass axis is fixed for the pencil; along it, k of point t + m TP (pencils past the edge: index 0)
+    const int64_t col = tl.valid ? tl.col : 0;
+    const int64_t i1 = fdiv64(col, g.fn2), i2 = col - i1 * g.n2;
+    const T k1 = g.n1 > 1 ? g.kvec[g.k1ofs + i1] : T(0);
+    const T k2 = g.n2 > 1 ? g.kvec[g.k2ofs + i2] : T(0);
+    const bool dc_pencil = (i1 | i2) == 0;
+    T kpre[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) kpre[m] = g.kvec[t + m * TP];
+    __syncthreads();
+    reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
+    const cx<T> zero{T(0), T(0)};
+    if constexpr (OP == 0) {
+        mag_store<T, N, E, NC>(v, a.w1, tl, V);
+    }
+    const T ar = a.alpha_r, inv_ai = T(1) / a.alpha_i;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const T k0 = kpre[m];
+        if constexpr (OP == 0 || OP == 2) {
+            const cx<T> f[3] = {v[0][m], v[1][m], v[2][m]};
+            cx<T> o[3];
+            cross_ik(k0, k1, k2, f, o);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c][m] = o[c];
+        } else {
+            const cx<T> y[3] = {v[3][m], v[4][m], v[5][m]};
+            cx<T> f[3];
+            cross_ik(k0, k1, k2, y, f);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] = f[c] + v[c][m];
+            const T kk = k0 * k0 + k1 * k1 + k2 * k2;
+            const cx<T> gs = green_scalar(kk, ar, inv_ai, T(1));  // -i alpha_i / (k^2 - alpha)
+            const cx<T> gl = green_longitudinal(ar, inv_ai, T(1));  // i alpha_i / alpha
+            cx<T> b[3];
+            if (dc_pencil && t + m * TP == 0) {  // k = 0: the first nd components count as longitudinal
+#pragma unroll
+                for (int c = 0; c < 3; ++c) b[c] = (c < a.nd ? gl : gs) * f[c];
+            } else {
+                const cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
+                const cx<T> q = (gl - gs) * scal(proj, T(1) / kk);
+                b[0] = gs * f[0] + scal(q, k0);
+                b[1] = gs * f[1] + scal(q, k1);
+                b[2] = gs * f[2] + scal(q, k2);
+            }
+            cx<T> eh[3] = {zero, zero, zero};
+            if (tl.valid) {
+                const cx<T>* w1t = a.w1 + tl.tb;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) eh[c] = ldg<true>(w1t + c * V, tl.toff + m * tl.mstep);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) b[c] = b[c] - eh[c];
+            cx<T> cb[3];
+            cross_ik(k0, k1, k2, b, cb);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[c][m] = b[c];
+                v[3 + c][m] = cb[c];
+            }
+        }
+    }
+    reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
+    mag_store<T, N, E, NC>(v, dst, tl, V);
+}
+
+// pw passes along the last (contiguous) grid axis: inverse FFT of NI components of `src` (the last inverse axis:
+// scaled by 1/V), voxel-wise products, then
+//   STAGE 1 (NI = 3: curl E): [Z1; Y1] = [chi_EE E + chi_HE curl E + s; chi_EH E + chi_HH curl E], forward FFT, to dst
+//                             (6 components);
+//   STAGE 2 (NI = 6: [B; curl B]): Z2 = chi_EE B + chi_HE curl B to w1 (real space), Y2 = chi_EH B + chi_HH curl B,
+//                             forward FFT, to dst (3 components);
+//   STAGE 3 (NI = 3: curl Y2): d = Z2 + curl Y2, E' = E + d to Eout, FP64 partials of |d|^2, |E'|^2 per tile.
+template <typename T, int N, int E, int STAGE>
+__global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
+    constexpr int TP = N / E;
+    constexpr int LTP = ilog2(TP);
+    constexpr int NI = STAGE == 2 ? 6 : 3;
+    extern __shared__ unsigned char smem_raw[];
+    __shared__ double red[16];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* xbuf = tw + N;
+    const int tid = threadIdx.x;
+    const int t = tid & (TP - 1);
+    const int p = tid >> LTP;
+    cx<T>* xb = xbuf + p * padded_len(N, E);
+    const MagTile tl = mag_tile<T, N, E>(g, p, t);
+    const int64_t V = a.V;
+    const cx<T> zero{T(0), T(0)};
+    load_twiddles<T, N, E>(tw, g.tw, tid, blockDim.x);
+    cx<T> x[NI][E];
+    mag_load<T, N, E, NI>(x, src, tl, V);
+    __syncthreads();
+    reg_fft<T, N, E, +1, NI>(x, t, tw, xb, 1);
+    const T sc = g.scale;
+    if constexpr (STAGE == 1) {
+        cx<T> y[6][E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            cx<T> e[3] = {zero, zero, zero}, zz[3] = {zero, zero, zero}, yy[3] = {zero, zero, zero};
+            const cx<T> c[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
+            if (tl.valid) {
+                const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
+                const cx<T>* Et = a.E + tl.tb;
+                const cx<T>* st = a.s + tl.tb;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    e[i] = ldg<true>(Et + i * V, tl.toff + m * tl.mstep);
+                    zz[i] = ldg<true>(st + i * V, tl.toff + m * tl.mstep);
+                }
+                mag_mul_add(a.ee, vox, V, e, zz);
+                mag_mul_add(a.he, vox, V, c, zz);
+                mag_mul_add(a.eh, vox, V, e, yy);
+                mag_mul_add(a.hh, vox, V, c, yy);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                y[i][m] = zz[i];
+                y[3 + i][m] = yy[i];
+            }
+        }
+        reg_fft<T, N, E, -1, 6>(y, t, tw, xb, 1);
+        mag_store<T, N, E, 6>(y, dst, tl, V);
+    } else if constexpr (STAGE == 2) {
+        cx<T> y[3][E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            cx<T> zz[3] = {zero, zero, zero}, yy[3] = {zero, zero, zero};
+            const cx<T> b[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
+            const cx<T> cb[3] = {scal(x[3][m], sc), scal(x[4][m], sc), scal(x[5][m], sc)};
+            if (tl.valid) {
+                const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
+                mag_mul_add(a.ee, vox, V, b, zz);
+                mag_mul_add(a.he, vox, V, cb, zz);
+                mag_mul_add(a.eh, vox, V, b, yy);
+                mag_mul_add(a.hh, vox, V, cb, yy);
+                cx<T>* zt = a.w1 + tl.tb;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) stg<true>(zt + i * V, tl.toff + m * tl.mstep, zz[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) y[i][m] = yy[i];
+        }
+        reg_fft<T, N, E, -1, 3>(y, t, tw, xb, 1);
+        mag_store<T, N, E, 3>(y, dst, tl, V);
+    } else {
+        double dd = 0.0, ee = 0.0;
+        if (tl.valid) {
+            const cx<T>* Et = a.E + tl.tb;
+            const cx<T>* zt = a.w1 + tl.tb;
+            cx<T>* Eo = a.Eout + tl.tb;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const uint32_t bo = tl.toff + m * tl.mstep;
+                    const cx<T> d = ldg<true>(zt + i * V, bo) + scal(x[i][m], sc);
+                    const cx<T> en = ldg<true>(Et + i * V, bo) + d;
+                    stg<true>(Eo + i * V, bo, en);
+                    dd += norm2_f64(d);
+                    ee += norm2_f64(en);
+                }
+            }
+        }
+        dd = block_sum(dd, red);
+        ee = block_sum(ee, red);
+        if (threadIdx.x == 0) {
+            a.partials[2 * blockIdx.x] = dd;
+            a.partials[2 * blockIdx.x + 1] = ee;
+        }
+    }
+}
+
+}  // namespace mmx

@@ -122,6 +122,7 @@ class Solution:
         self.__fields = [BE.allocate_array(fill_value=0) for _ in range(3)]
         self.__cur = 0
         self.__pending = None
+        self.__pending_version = None  # version of the pending iteration's input buffer when it was launched
         self.__work = BE.allocate_array(fill_value=0)  # first pass of the next iteration (FFT along axis -1)
         self.__work_valid = False
         self.__norms_dev = [torch.zeros(2, dtype=torch.float64, device=BE.device) for _ in range(2)]
@@ -611,6 +612,13 @@ class Solution:
         return self.relative_residue
 
     # ------------------------------------------------------------------ iteration (solver.py:1049-1098)
+    def __check_pending(self):
+        """An iteration launched ahead is void if its input, the current field, was changed in place since then
+        (e.g. through ``field_array``): torch counts in-place writes in the tensor's version; the kernels, which
+        write through raw pointers, do not change it."""
+        if self.__pending is not None and self.__fields[self.__cur]._version != self.__pending_version:
+            self.__invalidate_work()
+
     def __invalidate_work(self):
         """The work buffer no longer holds the first pass of the current field, and an iteration launched ahead
         is void (the field, the source, the medium or alpha changed)."""
@@ -634,6 +642,7 @@ class Solution:
         The next iteration is enqueued before this one's norms are read (assuming acceptance, which holds for
         nearly every iteration), so the device never waits for the host's decision; it is discarded when the update
         is rejected."""
+        self.__check_pending()
         if self.__magnetic:
             return self.__magnetic_step()
         self.__iteration += 1
@@ -650,6 +659,7 @@ class Solution:
         self.__work_valid = True
         # completes this iteration's successor on speculation: input `out`, output the third buffer
         self.__pending = self.__launch_iteration(out, 3 - cur - out, 1 - k)
+        self.__pending_version = self.__fields[out]._version
         ev.synchronize()  # waits for this iteration only
         d2, e2 = self.__norms_host[k].tolist()
         current = math.sqrt(d2)
@@ -672,20 +682,20 @@ class Solution:
         FP64 (``solver.py:1070-1082``)."""
         self.__mag.iterate(self.__fields[cur], self.__fields[nxt], norms)
 
-    def __magnetic_launch(self, cur: int, nxt: int) -> torch.Tensor:
-        """Enqueue one magnetic iteration; returns the device tensor that will hold its two squared norms.
+    def __magnetic_record(self, cur: int, nxt: int, k: int):
+        """Enqueue one magnetic iteration from field buffer cur into nxt, its two squared norms into device slot k.
 
-        With ``magnetic_graphs``, the iteration's launches are recorded once per (input, output) buffer pair in a HIP
+        With ``magnetic_graphs``, the iteration's launches are recorded once per (input, output, norm slot) in a HIP
         graph (torch.cuda.CUDAGraph, one memory pool for all of them) and replayed. The graphs hold the operator's
         tensors, alpha and the plan's scratch address, so they are recorded again when the operator changes (new
         alpha or source) or the plan's scratch buffer has been reallocated. The first iteration after such a change
         runs eagerly, which also creates every lazily built constant outside the graph's memory pool."""
         if self.__mag_norms is None:
-            self.__mag_norms = torch.zeros(2, dtype=torch.float64, device=self.__BE.device)
-        norms = self.__mag_norms
+            self.__mag_norms = [torch.zeros(2, dtype=torch.float64, device=self.__BE.device) for _ in range(2)]
+        norms = self.__mag_norms[k]
         if not self.magnetic_graphs:
             self.__magnetic_compute(cur, nxt, norms)
-            return norms
+            return
         plan = self.__BE.plan
         state = (self.__mag.version, plan.get_param('scratch_gen'))
         if state != self.__mag_graph_state:
@@ -693,8 +703,8 @@ class Solution:
             self.__mag_pool = None
             self.__mag_graph_state = state
             self.__magnetic_compute(cur, nxt, norms)  # eager
-            return norms
-        key = (cur, nxt, self.__fields[cur].data_ptr(), self.__fields[nxt].data_ptr())
+            return
+        key = (cur, nxt, k, self.__fields[cur].data_ptr(), self.__fields[nxt].data_ptr())
         g = self.__mag_graphs.get(key)
         if g is None:
             g = torch.cuda.CUDAGraph()
@@ -705,23 +715,38 @@ class Solution:
             if plan.get_param('scratch_gen') != state[1]:  # recording grew the scratch: the graph is stale
                 self.__mag_graph_state = None
                 self.__magnetic_compute(cur, nxt, norms)
-                return norms
+                return
             self.__mag_graphs[key] = g
         g.replay()
-        return norms
+
+    def __magnetic_launch(self, cur: int, nxt: int, k: int):
+        """Enqueue one magnetic iteration (cur -> nxt) with its norms copied to pinned host slot k behind an event.
+        Returns (nxt, k, event)."""
+        self.__magnetic_record(cur, nxt, k)
+        self.__norms_host[k].copy_(self.__mag_norms[k], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.__BE.device))
+        return nxt, k, ev
 
     def __magnetic_step(self) -> bool:
-        """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``)."""
+        """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``). As for other
+        media, the next iteration is enqueued (assuming acceptance) before this one's norms are read."""
         self.__iteration += 1
         self.__relative_residue = None
         cur = self.__cur
-        nxt = (cur + 1) % 3
-        d2, e2 = self.__magnetic_launch(cur, nxt).tolist()
+        if self.__pending is not None:
+            out, k, ev = self.__pending
+        else:
+            out, k, ev = self.__magnetic_launch(cur, (cur + 1) % 3, 0)
+        self.__pending = self.__magnetic_launch(out, 3 - cur - out, 1 - k)
+        self.__pending_version = self.__fields[out]._version
+        ev.synchronize()  # waits for this iteration only
+        d2, e2 = self.__norms_host[k].tolist()
         current = math.sqrt(d2)
         with np.errstate(divide='ignore', invalid='ignore'):
             rel = np.float64(current) / np.float64(self.__previous_update_norm)
         if rel < 1:
-            self.__cur = nxt
+            self.__cur = out
             self.__field_fp64 = None
             self.__previous_update_norm = current
             self.__field_norm = math.sqrt(e2)
@@ -731,6 +756,7 @@ class Solution:
         log.info(f'Increasing the imaginary part of alpha from {self.__alpha.imag:0.3g} to {alpha_new.imag:0.3g}.')
         self.__mag.set_alpha(alpha_new)
         self.__alpha = alpha_new
+        self.__pending = None  # computed with the old alpha
         return False
 
     # ------------------------------------------------------------------ FP64 refinement

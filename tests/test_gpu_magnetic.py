@@ -316,3 +316,71 @@ def test_magnetic_fine_grid_extreme_alpha_matches_oracle():
         Eg = sol.field_array.cpu().numpy()
         err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
         assert err <= 2e-5, (it, err)
+
+
+@pytest.mark.parametrize('shape', [(64, 128), (128, 64), (64, 20, 64), (128, 12, 256)], ids=lambda s: 'x'.join(map(str, s)))
+def test_magnetic_fused_passes_match_oracle_and_unfused(shape):
+    """Grids whose first and last axes have fused magnetic passes (complex64, 64..1024 points): per-voxel anisotropic
+    permittivity, a constant 3x3 permeability and per-voxel bi-isotropy xi and zeta, so every susceptibility kind
+    reaches the pass kernels. The first update equals the unfused kernel sequence (register kernels off) and the
+    composed operator (MagneticOperator.update) to 2e-6, and 10 iterates follow MagneticBornOracle within 2e-5."""
+    import scipy.constants as const
+    import torch
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    from oracle import born_oracle as ob
+    print(f'magnetic fused passes {shape}', flush=True)
+    rng = np.random.default_rng(21)
+    nd = len(shape)
+    pitch, wavelength = 0.1e-6, 0.5e-6
+    k0 = 2 * np.pi / wavelength
+    grid = mm.Grid(list(shape), pitch)
+    a = rng.standard_normal((3, *shape))
+    a /= np.linalg.norm(a, axis=0)
+    eye = np.eye(3).reshape(3, 3, *([1] * nd))
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2 * eye + 0.2 * a[:, None] * a[None, :]
+    m = rng.standard_normal((3, 3)) * 0.1
+    mu = (np.eye(3) * 1.2 + m @ m.T).reshape(3, 3, *([1] * nd))
+    xi = 0.05 * (rng.random(shape) + 1j * rng.random(shape))
+    zeta = 0.03 * rng.random(shape)
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[(slice(0, 1),) + (slice(None),) * (nd - 1) + (slice(4, 5),)] = 1.0
+    J[(slice(1, 2),) + (slice(None),) * (nd - 1) + (slice(4, 5),)] = 0.3j
+    bound = LinearBound(grid, thickness=0.4e-6)
+    sol = mm.Solution(grid, vacuum_wavelength=wavelength, current_density=J, epsilon=eps, mu=mu, xi=xi, zeta=zeta,
+                      bound=bound, dtype=np.complex64)
+    assert sol.magnetic
+    mag = sol._Solution__mag
+    g = torch.Generator(device=mag.s.device).manual_seed(3)
+    E0 = (torch.randn((3, 1, *shape), generator=g, device=mag.s.device, dtype=torch.float32)
+          + 1j * torch.randn((3, 1, *shape), generator=g, device=mag.s.device, dtype=torch.float32)) * 1e-3
+    E0 = E0.to(mag.s.dtype)
+    norms = torch.zeros(2, dtype=torch.float64, device=E0.device)
+    E_fused = torch.empty_like(E0)
+    mag.iterate(E0, E_fused, norms)
+    n_fused = norms.clone()
+    d_composed = mag.update(E0)
+    mag.plan.set_param('reg_fft', 0)
+    E_unfused = torch.empty_like(E0)
+    mag.iterate(E0, E_unfused, norms)
+    mag.plan.set_param('reg_fft', 1)
+    d_f, d_u = E_fused - E0, E_unfused - E0
+    rel = float(torch.linalg.vector_norm(d_f - d_u) / torch.linalg.vector_norm(d_u))
+    assert rel <= 2e-6, rel
+    rel = float(torch.linalg.vector_norm(d_f - d_composed) / torch.linalg.vector_norm(d_composed))
+    assert rel <= 2e-6, rel
+    npt.assert_allclose(n_fused.cpu().numpy(), norms.cpu().numpy(), rtol=1e-5)
+    eps_b = eps + bound.electric_susceptibility * eye
+    alpha, beta = sol.alpha, sol.beta
+    _, _, ops, _, _ = ob.magnetic_setup(shape, pitch, k0, eps_b, xi, zeta, mu, np.asarray(bound.thickness).ravel(),
+                                        alpha_beta=(alpha, beta))
+    S = J.reshape(3, 1, *shape) * (-1j * k0 * const.c * const.mu_0)
+    k = ob.k_vectors(shape, np.array([pitch] * nd) * k0)
+    o = ob.MagneticBornOracle(ops, S * (1j / alpha.imag / beta), alpha, beta, k)
+    for it in range(10):
+        sol.step()
+        o.step()
+        assert sol.alpha == o.alpha
+        Eg = sol.field_array.cpu().numpy()
+        err = np.linalg.norm(Eg - o.E) / np.linalg.norm(o.E)
+        assert err <= 2e-5, (it, err)
