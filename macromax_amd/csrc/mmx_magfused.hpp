@@ -121,6 +121,9 @@ __global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> 
     T kpre[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) kpre[m] = g.kvec[t + m * TP];
+    // OP 1: E^ is requested before the forward FFT, so that it arrives during it
+    cx<T> eh[OP == 1 ? 3 : 1][OP == 1 ? E : 1];
+    if constexpr (OP == 1) mag_load<T, N, E, 3>(eh, a.w1, tl, V);
     __syncthreads();
     reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
     const cx<T> zero{T(0), T(0)};
@@ -157,14 +160,8 @@ __global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> 
                 b[1] = gs * f[1] + scal(q, k1);
                 b[2] = gs * f[2] + scal(q, k2);
             }
-            cx<T> eh[3] = {zero, zero, zero};
-            if (tl.valid) {
-                const cx<T>* w1t = a.w1 + tl.tb;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) eh[c] = ldg<true>(w1t + c * V, tl.toff + m * tl.mstep);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) b[c] = b[c] - eh[c];
+            for (int c = 0; c < 3; ++c) b[c] = b[c] - eh[c][m];
             cx<T> cb[3];
             cross_ik(k0, k1, k2, b, cb);
 #pragma unroll
@@ -204,6 +201,14 @@ __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a
     load_twiddles<T, N, E>(tw, g.tw, tid, blockDim.x);
     cx<T> x[NI][E];
     mag_load<T, N, E, NI>(x, src, tl, V);
+    // field operands of the voxel-wise products, requested before the inverse FFT so that they arrive during it:
+    // E and s (stage 1), Z2 and E (stage 3)
+    constexpr int NP = STAGE == 2 ? 1 : 3;
+    cx<T> f1[NP][STAGE == 2 ? 1 : E], f2[NP][STAGE == 2 ? 1 : E];
+    if constexpr (STAGE != 2) {
+        mag_load<T, N, E, 3>(f1, STAGE == 1 ? a.E : a.w1, tl, V);
+        mag_load<T, N, E, 3>(f2, STAGE == 1 ? a.s : a.E, tl, V);
+    }
     __syncthreads();
     reg_fft<T, N, E, +1, NI>(x, t, tw, xb, 1);
     const T sc = g.scale;
@@ -211,17 +216,11 @@ __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a
         cx<T> y[6][E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            cx<T> e[3] = {zero, zero, zero}, zz[3] = {zero, zero, zero}, yy[3] = {zero, zero, zero};
+            const cx<T> e[3] = {f1[0][m], f1[1][m], f1[2][m]};
+            cx<T> zz[3] = {f2[0][m], f2[1][m], f2[2][m]}, yy[3] = {zero, zero, zero};
             const cx<T> c[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
             if (tl.valid) {
                 const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
-                const cx<T>* Et = a.E + tl.tb;
-                const cx<T>* st = a.s + tl.tb;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    e[i] = ldg<true>(Et + i * V, tl.toff + m * tl.mstep);
-                    zz[i] = ldg<true>(st + i * V, tl.toff + m * tl.mstep);
-                }
                 mag_mul_add(a.ee, vox, V, e, zz);
                 mag_mul_add(a.he, vox, V, c, zz);
                 mag_mul_add(a.eh, vox, V, e, yy);
@@ -260,16 +259,14 @@ __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a
     } else {
         double dd = 0.0, ee = 0.0;
         if (tl.valid) {
-            const cx<T>* Et = a.E + tl.tb;
-            const cx<T>* zt = a.w1 + tl.tb;
             cx<T>* Eo = a.Eout + tl.tb;
 #pragma unroll
             for (int m = 0; m < E; ++m) {
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
                     const uint32_t bo = tl.toff + m * tl.mstep;
-                    const cx<T> d = ldg<true>(zt + i * V, bo) + scal(x[i][m], sc);
-                    const cx<T> en = ldg<true>(Et + i * V, bo) + d;
+                    const cx<T> d = f1[i][m] + scal(x[i][m], sc);
+                    const cx<T> en = f2[i][m] + d;
                     stg<true>(Eo + i * V, bo, en);
                     dd += norm2_f64(d);
                     ee += norm2_f64(en);
