@@ -666,6 +666,47 @@ def accuracy_benchmark(n, dev, E_plain, plain_residue, plain_iterations, log):
     return out
 
 
+def stream_ceiling(n, dev, must_move_update, reps=20):
+    """The bandwidth of kernels that only stream the planes of the passes (macromax_amd/csrc/mmx_stream_probe.hip:
+    rows of 256 complex64 values, non-temporal, no arithmetic): 3 read + 3 written planes (a plain or Green pass of three
+    components), 10 + 6 (the update pass with uniaxial chi), 6 + 6; and the iteration time the passes would take at
+    those rates (3 passes of 48 B at the 3 + 3 rate, the update pass's bytes at the 10 + 6 rate)."""
+    import ctypes
+    import torch
+    from macromax_amd import build as _b
+    if not os.path.exists(_b.PROBE_LIB_PATH) or n ** 3 % 2048:
+        return None
+    lib = ctypes.CDLL(_b.PROBE_LIB_PATH)
+    lib.mmx_stream_probe.restype = ctypes.c_int
+    lib.mmx_stream_probe.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+    V = n ** 3
+    src = torch.zeros(10 * V, dtype=torch.complex64, device=dev)
+    dst = torch.zeros(6 * V, dtype=torch.complex64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+    for nr, nw in ((3, 3), (6, 6), (10, 6)):
+        call = lambda: lib.mmx_stream_probe(nr, nw, V, src.data_ptr(), dst.data_ptr(), stream.cuda_stream)
+        for _ in range(3):
+            if call() != 0:
+                return None
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            call()
+        e1.record(stream)
+        e1.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3 / reps
+        out[f'rows_{nr}r{nw}w_GBps'] = round((nr + nw) * 8 * V / t / 1e9, 1)
+    del src, dst
+    r33, r106 = out['rows_3r3w_GBps'] * 1e9, out['rows_10r6w_GBps'] * 1e9
+    floor = V * (3 * 48 / r33 + must_move_update / r106)
+    out['iteration_floor_ms'] = round(floor * 1e3, 4)
+    out['what'] = ('streaming-only kernels with the passes\' plane counts; iteration_floor_ms = the axis-0, Green and '
+                   'update passes\' must-move bytes at those rates (the norm reduction excluded)')
+    return out
+
+
 def measured_traffic(n):
     """HBM bytes per voxel-iteration from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (256^3 only)."""
     path = os.path.join(ROOT, 'profiles', 'born_iterate_256cubed_aniso.md')
@@ -769,6 +810,11 @@ def main():
                 'alg_bytes_per_voxel_iter': b_alg,
                 'alg_equivalent_GBps': round(b_alg * V / gpu_per_iter / 1e9, 1),
                 'alg_equivalent_frac': round(b_alg * V / gpu_per_iter / 1e9 / HBM_PEAK_GBS, 4)}
+    ceiling = stream_ceiling(n, dev, b_must - 3 * 48)
+    if ceiling is not None:
+        ceiling['fraction_of_floor'] = round(ceiling['iteration_floor_ms'] / (gpu_per_iter * 1e3), 4)
+        roofline['stream_ceiling'] = ceiling
+        log(f"stream ceiling: {ceiling}")
     if traffic is not None:
         roofline['traffic_bytes_per_voxel_iter'] = traffic
         roofline['traffic_GBps'] = round(traffic * V / gpu_per_iter / 1e9, 1)
