@@ -134,3 +134,15 @@ def test_no_oracle_or_cpu_fallback_in_the_product():
                 text = open(os.path.join(dirpath, f)).read()
                 assert 'oracle' not in text.replace('oracle_', ''), f
                 assert 'numpy.fft' not in text and 'np.fft.fft' not in text and 'torch.fft' not in text, f
+
+
+def test_stream_probe_library_exports_its_entry_point():
+    """The streaming-ceiling probe of bench.py is built beside the library and exports mmx_stream_probe (loaded with
+    ctypes; no GPU call)."""
+    import ctypes
+    import os
+    from macromax_amd import build
+    if not os.path.exists(build.PROBE_LIB_PATH):
+        pytest.skip('libmmx_stream_probe.so not built (python -m macromax_amd.build)')
+    lib = ctypes.CDLL(build.PROBE_LIB_PATH)
+    assert hasattr(lib, 'mmx_stream_probe')
