@@ -185,6 +185,20 @@ def test_in_place_field_edit_voids_the_iteration_launched_ahead():
         assert err <= 1e-10, (it, err)
 
 
+def test_adaptive_stop_criterion():
+    """macromax_amd.utils.Adaptive (utils/stop_criterion.py:7-52) as the solve callback: the solve stops at a reading
+    below the target residue, within a few iterations of where the every-iteration rule stops."""
+    import bench
+    from macromax_amd.utils import Adaptive
+    # the 128^2 disc medium of the batched benchmark (absorbing boundary; about 900 iterations to 1e-6)
+    plain = bench.disc_problem(128, 7).solve(lambda s: s.residue > 1e-6 and s.iteration < 10000)
+    adaptive = bench.disc_problem(128, 7).solve(Adaptive(1e-6, 10000))
+    assert plain.residue < 1e-6 and plain.iteration < 10000
+    assert adaptive.residue < 1e-6
+    assert plain.iteration <= adaptive.iteration <= int(1.25 * plain.iteration) + 2, (plain.iteration,
+                                                                                       adaptive.iteration)
+
+
 def _oracle_from_solution(sol):
     chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
     s = sol.source_normalized.cpu().numpy().astype(np.complex128)
