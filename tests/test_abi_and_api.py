@@ -81,8 +81,10 @@ def test_grid_matches_reference_behaviour():
     g = Grid([5, 10, 20], 100e-9)
     npt.assert_array_equal(g.shape, [5, 10, 20])
     npt.assert_allclose((g * 1e7).step, [1, 1, 1])
-    npt.assert_allclose(Grid(5).f[0].ravel(), [0, 1 / 5, 2 / 5, -2 / 5, -1 / 5])
-    npt.assert_allclose(Grid(5, 2).k[0].ravel(), np.array([0, 1 / 10, 2 / 10, -2 / 10, -1 / 10]) * 2 * np.pi)
+    # a grid defined by scalars is the sequence of its coordinates
+    npt.assert_allclose(np.asarray(Grid(5).f), [0, 1 / 5, 2 / 5, -2 / 5, -1 / 5])
+    npt.assert_allclose(np.asarray(Grid(5, 2).k), np.array([0, 1 / 10, 2 / 10, -2 / 10, -1 / 10]) * 2 * np.pi)
+    assert Grid(5).f[1] == 1 / 5
     npt.assert_allclose(Grid([5, 2]).f[0], np.array([[0], [1 / 5], [2 / 5], [-2 / 5], [-1 / 5]]))
     r = np.arange(10) * 0.5 - 1
     g1 = Grid.from_ranges(r)
