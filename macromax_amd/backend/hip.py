@@ -575,6 +575,15 @@ class BackEndHip(BackEnd):
         F = self._field(field_array_ft)
         return F - self.longitudinal_projection_ft(F)
 
+    def longitudinal_projection(self, field_array) -> torch.Tensor:
+        """The longitudinal (curl-free) part of a vector field [3, 1, *grid] (``backend/__init__.py:582-591``): the
+        k-space projection between a forward and an inverse FFT (HIP kernels)."""
+        return self.convolve(self.longitudinal_projection_ft, field_array)
+
+    def transversal_projection(self, field_array) -> torch.Tensor:
+        """The transverse (divergence-free) part of a vector field (``backend/__init__.py:571-580``)."""
+        return self.convolve(self.transversal_projection_ft, field_array)
+
     def curl_ft(self, field_array_ft) -> torch.Tensor:
         """i k x F (``backend/__init__.py:483-509``).
 

@@ -269,3 +269,21 @@ def test_div_ft_and_div(BE):
     E = a[:, None, None, None, None] * phase[None, None]
     expected = 1j * np.dot(k, a) * phase[None, None]
     npt.assert_allclose(BE.asnumpy(BE.div(BE.astype(E))), expected, atol=2e-5 * np.abs(expected).max())
+
+
+def test_real_space_projections(BE):
+    """longitudinal_projection / transversal_projection (backend/__init__.py:571-591): the k-space projections between
+    an FFT pair. Against the oracle's projection under numpy.fft; L + T = F, div T = 0, curl L = 0, L(L(F)) = L(F)."""
+    from oracle import born_oracle as ob
+    rng = np.random.default_rng(8)
+    F = rng.standard_normal((3, 1, 5, 10, 20)) + 1j * rng.standard_normal((3, 1, 5, 10, 20))
+    L = BE.asnumpy(BE.longitudinal_projection(BE.astype(F.copy())))
+    T = BE.asnumpy(BE.transversal_projection(BE.astype(F.copy())))
+    k = ob.k_vectors((5, 10, 20), [1.0, 1.0, 1.0])
+    L_ref = np.fft.ifftn(ob.longitudinal_projection_ft(np.fft.fftn(F, axes=(2, 3, 4)), k), axes=(2, 3, 4))
+    scale = np.abs(F).max()
+    npt.assert_allclose(L, L_ref, atol=2e-5 * scale)
+    npt.assert_allclose(L + T, F, atol=2e-5 * scale)
+    npt.assert_allclose(BE.asnumpy(BE.div(BE.astype(T))), 0, atol=2e-4 * scale)
+    npt.assert_allclose(BE.asnumpy(BE.curl(BE.astype(L))), 0, atol=2e-4 * scale)
+    npt.assert_allclose(BE.asnumpy(BE.longitudinal_projection(BE.astype(L))), L, atol=2e-5 * scale)
