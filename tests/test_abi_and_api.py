@@ -148,3 +148,15 @@ def test_stream_probe_library_exports_its_entry_point():
         pytest.skip('libmmx_stream_probe.so not built (python -m macromax_amd.build)')
     lib = ctypes.CDLL(build.PROBE_LIB_PATH)
     assert hasattr(lib, 'mmx_stream_probe')
+
+
+def test_grid_import_paths_of_the_reference():
+    """Grid is reachable under the reference's import paths: macromax.utils, macromax.utils.ft and
+    macromax.utils.ft.grid (with MutableGrid), all the same class."""
+    import macromax_amd
+    from macromax_amd import grid
+    from macromax_amd.utils import Grid as G1
+    from macromax_amd.utils.ft import Grid as G2, MutableGrid as M2
+    from macromax_amd.utils.ft.grid import Grid as G3, MutableGrid as M3
+    assert macromax_amd.Grid is G1 is G2 is G3 is grid.Grid
+    assert M2 is M3 is grid.MutableGrid
