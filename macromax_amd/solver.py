@@ -128,6 +128,9 @@ class Solution:
         self.__work_valid = False
         self.__norms_dev = [torch.zeros(2, dtype=torch.float64, device=BE.device) for _ in range(2)]
         self.__norms_host = [torch.zeros(2, dtype=torch.float64, pin_memory=True) for _ in range(2)]
+        # the norms' copies to the host run on a stream of their own (default priority), so that the next
+        # iteration's kernels do not queue behind them
+        self.__copy_stream = torch.cuda.Stream(device=BE.device)
         self.__field_fp64 = None
         self.__BE64 = None
         self.__mag_graphs = {}  # (input buffer, output buffer) -> graph of one magnetic iteration
@@ -632,10 +635,21 @@ class Solution:
         BE = self.__BE
         BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
                         self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box)
-        self.__norms_host[k].copy_(self.__norms_dev[k], non_blocking=True)
-        ev = torch.cuda.Event()
-        ev.record(torch.cuda.current_stream(BE.device))
-        return dst, k, ev
+        return dst, k, self.__copy_norms(self.__norms_dev[k], k)
+
+    def __copy_norms(self, norms_dev: torch.Tensor, k: int) -> torch.cuda.Event:
+        """Copy an iteration's two norms to pinned host slot k once the launching stream has computed them; the copy
+        runs on the copy stream. Returns the event that marks its completion. Slot k is written again two
+        iterations later, after the host has waited for this event."""
+        done = torch.cuda.Event()
+        done.record(torch.cuda.current_stream(self.__BE.device))
+        self.__copy_stream.wait_event(done)
+        with torch.cuda.stream(self.__copy_stream):
+            self.__norms_host[k].copy_(norms_dev, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.__copy_stream)
+        norms_dev.record_stream(self.__copy_stream)  # not reused for other tensors before the copy ends
+        return ev
 
     def step(self) -> bool:
         """One Born iteration; returns True if the update was accepted.
@@ -724,10 +738,7 @@ class Solution:
         """Enqueue one magnetic iteration (cur -> nxt) with its norms copied to pinned host slot k behind an event.
         Returns (nxt, k, event)."""
         self.__magnetic_record(cur, nxt, k)
-        self.__norms_host[k].copy_(self.__mag_norms[k], non_blocking=True)
-        ev = torch.cuda.Event()
-        ev.record(torch.cuda.current_stream(self.__BE.device))
-        return nxt, k, ev
+        return nxt, k, self.__copy_norms(self.__mag_norms[k], k)
 
     def __magnetic_step(self) -> bool:
         """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``). As for other
