@@ -128,9 +128,17 @@ class Solution:
         self.__work_valid = False
         self.__norms_dev = [torch.zeros(2, dtype=torch.float64, device=BE.device) for _ in range(2)]
         self.__norms_host = [torch.zeros(2, dtype=torch.float64, pin_memory=True) for _ in range(2)]
-        # the norms' copies to the host run on a stream of their own (default priority), so that the next
-        # iteration's kernels do not queue behind them
-        self.__copy_stream = torch.cuda.Stream(device=BE.device)
+        # slot k's events: iteration done on the launch stream, norms copied to the host
+        self.__done_events = [torch.cuda.Event() for _ in range(2)]
+        self.__copy_events = [torch.cuda.Event() for _ in range(2)]
+        # On large grids (iterations of 0.2 ms and more) the norms' copies to the host run on a stream of their own
+        # (default priority), so that the next iteration's kernels do not queue behind them; on small grids, where
+        # the host's launches set the pace, the copy follows the iteration on its stream (fewer host calls).
+        self.__copy_stream = None
+        if int(np.prod(self.grid.shape)) * (3 if self.vectorial else 1) >= 1 << 24:
+            self.__copy_stream = torch.cuda.Stream(device=BE.device)
+            for t in self.__norms_dev:
+                t.record_stream(self.__copy_stream)  # not reused for other tensors before the copies end
         self.__field_fp64 = None
         self.__BE64 = None
         self.__mag_graphs = {}  # (input buffer, output buffer) -> graph of one magnetic iteration
@@ -638,17 +646,21 @@ class Solution:
         return dst, k, self.__copy_norms(self.__norms_dev[k], k)
 
     def __copy_norms(self, norms_dev: torch.Tensor, k: int) -> torch.cuda.Event:
-        """Copy an iteration's two norms to pinned host slot k once the launching stream has computed them; the copy
-        runs on the copy stream. Returns the event that marks its completion. Slot k is written again two
-        iterations later, after the host has waited for this event."""
-        done = torch.cuda.Event()
-        done.record(torch.cuda.current_stream(self.__BE.device))
+        """Copy an iteration's two norms to pinned host slot k once the launching stream has computed them (on the
+        copy stream where there is one). Returns the event that marks the copy's completion. Slot k and its events
+        are used again two iterations later, after the host has waited for this event."""
+        launch = torch.cuda.current_stream(self.__BE.device)
+        ev = self.__copy_events[k]
+        if self.__copy_stream is None:
+            self.__norms_host[k].copy_(norms_dev, non_blocking=True)
+            ev.record(launch)
+            return ev
+        done = self.__done_events[k]
+        done.record(launch)
         self.__copy_stream.wait_event(done)
         with torch.cuda.stream(self.__copy_stream):
             self.__norms_host[k].copy_(norms_dev, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record(self.__copy_stream)
-        norms_dev.record_stream(self.__copy_stream)  # not reused for other tensors before the copy ends
+        ev.record(self.__copy_stream)
         return ev
 
     def step(self) -> bool:
@@ -707,6 +719,9 @@ class Solution:
         runs eagerly, which also creates every lazily built constant outside the graph's memory pool."""
         if self.__mag_norms is None:
             self.__mag_norms = [torch.zeros(2, dtype=torch.float64, device=self.__BE.device) for _ in range(2)]
+            if self.__copy_stream is not None:
+                for t in self.__mag_norms:
+                    t.record_stream(self.__copy_stream)
         norms = self.__mag_norms[k]
         if not self.magnetic_graphs:
             self.__magnetic_compute(cur, nxt, norms)
