@@ -170,6 +170,21 @@ int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const 
                      double* norms, void* stream);
 
 /*
+ * mmx_born_iterate with its norm reduction on a second stream: the passes run on `stream`; the FP64 sum of their
+ * per-tile partials runs on `norm_stream` (a different stream of the same device) once `stream` has finished them,
+ * so that work enqueued on `stream` next (the following iteration) does not wait for it. The partials go to one of
+ * two buffers of the plan, chosen by `slot` (0 or 1), each with an event that the call records on `stream` and
+ * that norm_stream waits for: a call may reuse a slot only once the reduction of the previous call with that slot
+ * has run (alternate the slots and wait on norm_stream for call n before issuing call n + 2), since it overwrites
+ * the slot's partials and records its event again. Eout, W and norms as for mmx_born_iterate; norms is written on
+ * norm_stream.
+ * Replaces Solution.__iter__ (solver.py:1064-1098), as mmx_born_iterate.
+ */
+int mmx_born_iterate_split(mmx_plan_t plan, const void* E, void* Eout, void* W, const void* s, const void* chi,
+                           int ncomp, int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride,
+                           int64_t chi_bstride, double* norms, int slot, void* stream, void* norm_stream);
+
+/*
  * One Born iteration for a magnetic or bi-(an)isotropic medium (MacroMax Solution.__iter__ with the curl-term
  * susceptibility operator, solver.py:531-552 and 1064-1085; three components):
  *     chi X = chi_EE X + chi_HE curl X + curl(chi_EH X + chi_HH curl X),  d = chi (G (chi E + s) - E),  Eout = E + d,

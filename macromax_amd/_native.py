@@ -19,7 +19,7 @@ MMX_C128 = 1
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_register_fft_length', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
             'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_step_noalloc',
             'mmx_born_prologue', 'mmx_magnetic_iterate',
-            'mmx_born_iterate',
+            'mmx_born_iterate', 'mmx_born_iterate_split',
             'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_setup_sigma', 'mmx_setup_gain',
             'mmx_norm2', 'mmx_chi_mul',
             'mmx_longitudinal_projection_ft')
@@ -75,6 +75,9 @@ def _declare(L):
     L.mmx_born_prologue.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p]
     L.mmx_born_iterate.restype = c_int
     L.mmx_born_iterate.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
+    L.mmx_born_iterate_split.restype = c_int
+    L.mmx_born_iterate_split.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p,
+                                         c_int, c_p, c_p]
     L.mmx_batch_step.restype = c_int
     L.mmx_batch_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p,
                                  ctypes.c_double, c_i64, c_p]

@@ -225,17 +225,24 @@ class Plan:
         return W
 
     def born_iterate(self, E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch=1, s_bstride=0, chi_bstride=0,
-                     s_box=None):
-        """Finish the iteration begun in W (``mmx_born_iterate``): Eout = E + d, norms, W = next prologue."""
+                     s_box=None, norm_stream=None, slot=0):
+        """Finish the iteration begun in W (``mmx_born_iterate``): Eout = E + d, norms, W = next prologue.
+
+        With ``norm_stream`` (a torch stream of the same device other than the current one), the norms are reduced
+        on that stream once the current one has run the passes, from partial buffer ``slot`` (0 or 1) of the plan
+        (``mmx_born_iterate_split``): a slot may be used again once the reduction of its previous call has run."""
         self._born_checks(E, ((Eout, 'Eout'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
                           chi_bstride)
         if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
             raise ValueError('norms must be a float64 device tensor of 2*nbatch elements')
         self.set_source_support(*(s_box if s_box is not None else (None, None)))
-        _native.check(self._lib.mmx_born_iterate(self.handle, E.data_ptr(), Eout.data_ptr(), W.data_ptr(),
-                                                 s.data_ptr(), chi.data_ptr(), int(ncomp), int(chi_kind),
-                                                 coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride),
-                                                 norms.data_ptr(), _stream_ptr(self.device)), 'mmx_born_iterate')
+        args = (self.handle, E.data_ptr(), Eout.data_ptr(), W.data_ptr(), s.data_ptr(), chi.data_ptr(), int(ncomp),
+                int(chi_kind), coef.data_ptr(), int(nbatch), int(s_bstride), int(chi_bstride), norms.data_ptr())
+        if norm_stream is None:
+            _native.check(self._lib.mmx_born_iterate(*args, _stream_ptr(self.device)), 'mmx_born_iterate')
+        else:
+            _native.check(self._lib.mmx_born_iterate_split(*args, int(slot), _stream_ptr(self.device),
+                                                           norm_stream.cuda_stream), 'mmx_born_iterate_split')
         return norms
 
     def batch_step(self, EA, EB, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, state, track,
@@ -694,10 +701,10 @@ class BackEndHip(BackEnd):
                                          s_box=s_box)
 
     def born_iterate(self, E, Eout, W, s, chi, chi_kind: int, coef, norms, nbatch: int = 1, s_bstride: int = 0,
-                     chi_bstride: int = 0, s_box=None):
+                     chi_bstride: int = 0, s_box=None, norm_stream=None, slot: int = 0):
         ncomp = E.shape[-self.grid.ndim - 2]
         return self.__plan.born_iterate(E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride,
-                                        chi_bstride, s_box=s_box)
+                                        chi_bstride, s_box=s_box, norm_stream=norm_stream, slot=slot)
 
     def __str__(self) -> str:
         return f'BackEndHip(vectorial={self.vectorial}, {self.grid}, dtype={self.numpy_dtype}, device={self.__device})'

@@ -54,6 +54,10 @@ struct mmx_plan_s {
     void* scratch;   // FP64 partial sums
     size_t scratch_bytes;
     int64_t scratch_gen;  // number of (re)allocations of scratch: work captured in a graph holds its address
+    // mmx_born_iterate_split: two partial buffers, each with the event that marks its last pass on the launch stream
+    void* slot_partials[2];
+    size_t slot_bytes[2];
+    hipEvent_t slot_done[2];
     int device;
     size_t lds_max;  // device limit of shared memory per block
     // tuning (0 = automatic)
@@ -648,23 +652,58 @@ int prologue_impl(mmx_plan_s* p, const void* E, void* W, const void* s, const vo
 
 // One iteration from W (first pass already applied to E): remaining forward passes, Green, inverse passes, then
 // the fused last pass: Eout = E + chi_eff (W - E), norms, and W <- first pass of the next iteration on Eout.
+// Partial-sum buffer `slot` (0 or 1) of mmx_born_iterate_split, grown to `bytes`; its event is created with it.
+int ensure_slot(mmx_plan_s* p, int slot, size_t bytes) {
+    if (!p->slot_done[slot]) MMX_HIP_CHECK(hipEventCreateWithFlags(&p->slot_done[slot], hipEventDisableTiming));
+    if (p->slot_bytes[slot] >= bytes) return MMX_OK;
+    if (p->slot_partials[slot]) MMX_HIP_CHECK(hipFree(p->slot_partials[slot]));
+    p->slot_partials[slot] = nullptr;
+    p->slot_bytes[slot] = 0;
+    MMX_HIP_CHECK(hipMalloc(&p->slot_partials[slot], bytes));
+    p->slot_bytes[slot] = bytes;
+    ++p->scratch_gen;
+    return MMX_OK;
+}
+
+// Where an iteration's norm partials go and where they are reduced: the plan's scratch and the launch stream, or
+// (norm_stream set) partial slot `slot` and norm_stream, once the launch stream has finished the passes.
+struct NormRoute {
+    hipStream_t norm_stream = nullptr;
+    int slot = 0;
+};
+
+int partials_for(mmx_plan_s* p, const NormRoute& r, size_t bytes, double** partials) {
+    int rc = r.norm_stream ? ensure_slot(p, r.slot, bytes) : ensure_scratch(p, bytes);
+    if (rc) return rc;
+    *partials = reinterpret_cast<double*>(r.norm_stream ? p->slot_partials[r.slot] : p->scratch);
+    return MMX_OK;
+}
+
+int reduce_routed(mmx_plan_s* p, const NormRoute& r, const double* partials, int64_t tiles, int64_t nbatch,
+                  double* norms, hipStream_t stream) {
+    if (!r.norm_stream) return reduce_norms(partials, tiles, nbatch, norms, stream);
+    MMX_HIP_CHECK(hipEventRecord(p->slot_done[r.slot], stream));
+    MMX_HIP_CHECK(hipStreamWaitEvent(r.norm_stream, p->slot_done[r.slot], 0));
+    return reduce_norms(partials, tiles, nbatch, norms, r.norm_stream);
+}
+
 template <typename T>
 int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void* s, const void* chi, int C,
                  int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride, int64_t chi_bstride,
-                 double* norms, hipStream_t stream, const unsigned* state = nullptr) {
+                 double* norms, hipStream_t stream, const unsigned* state = nullptr, NormRoute route = NormRoute()) {
     int rc = born_check<T>(p, C, chi_kind);
     if (rc) return rc;
     if (nbatch <= 0) return MMX_OK;
     if (!fused_schedule_fits<T>(p, C)) {
         const int64_t tiles = pointwise_blocks(p->V);
-        rc = ensure_scratch(p, (size_t)(2 * tiles * nbatch) * sizeof(double));
+        double* partials = nullptr;
+        rc = partials_for(p, route, (size_t)(2 * tiles * nbatch) * sizeof(double), &partials);
         if (rc) return rc;
-        double* partials = reinterpret_cast<double*>(p->scratch);
         PassArgs<T> a = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
         a.state = state;
         rc = unfused_core<T>(p, a, Wv, nbatch, stream);
         if (!rc) rc = unfused_prologue<T>(p, a, Wv, nbatch, 1, stream);
-        if (!rc) rc = reduce_norms(partials, tiles, nbatch, norms, stream);
+        if (!rc) rc = reduce_routed(p, route, partials, tiles, nbatch, norms, stream);
         return rc;
     }
     const int d = p->naxes;
@@ -674,9 +713,9 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
     rc = last_pass_tiles<T>(p, inner, C, nbatch, last_kind, tl);
     if (rc) return rc;
     const int64_t tiles_last = tl.tiles_per_problem;
-    rc = ensure_scratch(p, (size_t)(2 * tiles_last * nbatch) * sizeof(double));
+    double* partials = nullptr;
+    rc = partials_for(p, route, (size_t)(2 * tiles_last * nbatch) * sizeof(double), &partials);
     if (rc) return rc;
-    double* partials = reinterpret_cast<double*>(p->scratch);
     PassArgs<T> base = born_base<T>(p, E, Eout, s, chi, C, chi_kind, coef, s_bstride, chi_bstride, partials, norms);
     base.state = state;
     cx<T>* W = reinterpret_cast<cx<T>*>(Wv);
@@ -726,7 +765,7 @@ int iterate_impl(mmx_plan_s* p, const void* E, void* Eout, void* Wv, const void*
         rc = launch_pass<T>(p, inner, a, nbatch, true, last_kind, stream);
         if (rc) return rc;
     }
-    return reduce_norms(partials, tiles_last, nbatch, norms, stream);
+    return reduce_routed(p, route, partials, tiles_last, nbatch, norms, stream);
 }
 
 // fixed_partials != nullptr: the per-tile partial sums go there (at most fixed_cap bytes) instead of the plan's
@@ -1245,6 +1284,10 @@ int mmx_plan_destroy(mmx_plan_t p) {
     if (p->kvec) (void)hipFree(p->kvec);
     if (p->scratch) (void)hipFree(p->scratch);
     if (p->fft_scratch) (void)hipFree(p->fft_scratch);
+    for (int q = 0; q < 2; ++q) {
+        if (p->slot_partials[q]) (void)hipFree(p->slot_partials[q]);
+        if (p->slot_done[q]) (void)hipEventDestroy(p->slot_done[q]);
+    }
     delete p;
     return MMX_OK;
 }
@@ -1405,6 +1448,25 @@ int mmx_born_iterate(mmx_plan_t p, const void* E, void* Eout, void* W, const voi
     return p->dtype == MMX_C64
                ? iterate_impl<float>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st)
                : iterate_impl<double>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms, st);
+}
+
+int mmx_born_iterate_split(mmx_plan_t p, const void* E, void* Eout, void* W, const void* s, const void* chi,
+                           int ncomp, int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride,
+                           int64_t chi_bstride, double* norms, int slot, void* stream, void* norm_stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !W || !s || !chi || !coef || !norms || !norm_stream) return fail(MMX_EINVAL, "null argument");
+    if (E == Eout || E == W || Eout == W) return fail(MMX_EINVAL, "E, Eout and W must be three different buffers");
+    if (slot != 0 && slot != 1) return fail(MMX_EINVAL, "slot must be 0 or 1");
+    if (norm_stream == stream) return fail(MMX_EINVAL, "norm_stream must differ from stream");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    NormRoute route;
+    route.norm_stream = reinterpret_cast<hipStream_t>(norm_stream);
+    route.slot = slot;
+    return p->dtype == MMX_C64
+               ? iterate_impl<float>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
+                                     st, nullptr, route)
+               : iterate_impl<double>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride,
+                                      norms, st, nullptr, route);
 }
 
 int mmx_batch_step(mmx_plan_t p, void* EA, void* EB, void* W, const void* s, const void* chi, int ncomp,

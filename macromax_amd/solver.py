@@ -641,9 +641,19 @@ class Solution:
         """Enqueue one fused iteration from field buffer src into dst, its norms into slot k (copied to pinned host
         memory behind an event). Returns (dst, k, event)."""
         BE = self.__BE
+        if self.__copy_stream is None:
+            BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
+                            self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box)
+            return dst, k, self.__copy_norms(self.__norms_dev[k], k)
+        # the norm reduction and its copy run on the copy stream, behind the iteration's passes (partial slot k)
         BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
-                        self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box)
-        return dst, k, self.__copy_norms(self.__norms_dev[k], k)
+                        self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box,
+                        norm_stream=self.__copy_stream, slot=k)
+        with torch.cuda.stream(self.__copy_stream):
+            self.__norms_host[k].copy_(self.__norms_dev[k], non_blocking=True)
+        ev = self.__copy_events[k]
+        ev.record(self.__copy_stream)
+        return dst, k, ev
 
     def __copy_norms(self, norms_dev: torch.Tensor, k: int) -> torch.cuda.Event:
         """Copy an iteration's two norms to pinned host slot k once the launching stream has computed them (on the

@@ -333,6 +333,72 @@ def test_prologue_iterate_matches_step(shape, C, kind, dtype, device):
     assert float(torch.linalg.norm(W - W_next) / torch.linalg.norm(W_next)) < tol
 
 
+@pytest.mark.parametrize('shape,C,kind,dtype', [((128, 128, 128), 3, 4, 'c64'), ((64, 96), 1, 1, 'c64'),
+                                                ((512,), 3, 3, 'c64'), ((13, 17, 11), 3, 3, 'c64'),
+                                                ((16, 20, 24), 3, 2, 'c128')],
+                         ids=lambda v: str(v))
+def test_iterate_split_matches_iterate(shape, C, kind, dtype, device):
+    """mmx_born_iterate_split (norm reduction on a second stream, partial slots 0 and 1 alternating) gives the
+    fields, work buffer and norms of mmx_born_iterate, bit for bit, over four iterations. As its contract asks,
+    call n + 2 is issued only after the reduction of call n has run (an event on the norm stream), as
+    Solution.step does."""
+    import torch
+    print(f'iterate split {shape} C={C} kind={kind} {dtype}', flush=True)
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, seed=7)
+    tdt = torch.complex64 if dtype == 'c64' else torch.complex128
+    rdt = torch.float32 if dtype == 'c64' else torch.float64
+    plan = _plan(shape, [0.37] * len(shape), tdt, device)
+    Et = torch.as_tensor(E[0], dtype=tdt, device=device).contiguous()
+    st = torch.as_tensor(s[0], dtype=tdt, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64 if tdt == torch.complex64 else np.complex128),
+                         device=device).contiguous()
+    coef = torch.tensor([alphas[0].real, alphas[0].imag, a[0], c[0].real, c[0].imag, 0.0], dtype=rdt, device=device)
+    norm_stream = torch.cuda.Stream(device=device)
+    runs = {}
+    for split in (False, True):
+        bufs = [Et.clone(), torch.empty_like(Et), torch.empty_like(Et)]
+        W = torch.empty_like(Et)
+        plan.born_prologue(bufs[0], W, st, ct, C, kind, coef)
+        norms = [torch.full((2,), -1.0, dtype=torch.float64, device=device) for _ in range(4)]
+        reduced = []
+        for i in range(4):
+            src, dst = bufs[i % 3], bufs[(i + 1) % 3]
+            if split:
+                if i >= 2:
+                    reduced[i - 2].synchronize()
+                plan.born_iterate(src, dst, W, st, ct, C, kind, coef, norms[i], norm_stream=norm_stream, slot=i % 2)
+                ev = torch.cuda.Event()
+                ev.record(norm_stream)
+                reduced.append(ev)
+            else:
+                plan.born_iterate(src, dst, W, st, ct, C, kind, coef, norms[i])
+        torch.cuda.synchronize()
+        runs[split] = ([b.cpu().numpy() for b in bufs], W.cpu().numpy(), [n.cpu().numpy() for n in norms])
+    for b0, b1 in zip(runs[False][0], runs[True][0]):
+        assert np.array_equal(b0, b1)
+    assert np.array_equal(runs[False][1], runs[True][1])
+    for n0, n1 in zip(runs[False][2], runs[True][2]):
+        assert np.array_equal(n0, n1) and np.all(n0 > 0)
+
+
+def test_iterate_split_rejects_bad_arguments(device):
+    """mmx_born_iterate_split refuses a slot other than 0 and 1 and a norm stream equal to the launch stream."""
+    import torch
+    from macromax_amd import _native
+    shape = (32, 32)
+    plan = _plan(shape, [0.5, 0.5], torch.complex64, device)
+    z = [torch.zeros(shape, dtype=torch.complex64, device=device) for _ in range(5)]
+    coef = torch.tensor([1.0, 1.0, 1.0, 0.0, 0.0, 0.0], dtype=torch.float32, device=device)
+    norms = torch.zeros(2, dtype=torch.float64, device=device)
+    with pytest.raises(_native.NativeError, match='slot'):
+        plan.born_iterate(z[0], z[1], z[2], z[3], z[4], 1, 1, coef, norms, norm_stream=torch.cuda.Stream(device=device),
+                          slot=2)
+    launch = torch.cuda.Stream(device=device)
+    with torch.cuda.stream(launch), pytest.raises(_native.NativeError, match='norm_stream'):
+        plan.born_iterate(z[0], z[1], z[2], z[3], z[4], 1, 1, coef, norms, norm_stream=launch, slot=0)
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize('shape,C,kind', [((64, 128, 64), 3, 3), ((20, 24, 40), 3, 2), ((48, 40), 1, 1),
                                           ((256,), 3, 1), ((256, 64), 3, 1), ((64, 128, 64), 3, 4)],
                          ids=lambda v: str(v))
