@@ -803,8 +803,9 @@ def main():
     roofline = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': round(achieved / HBM_PEAK_GBS, 4),
                 'traffic': None if traffic is None else round(traffic * V),
-                'kernel': 'mmx_born_iterate: one Born iteration = 4 pencil-pass kernels + FP64 norm reduction, '
-                          'timed with HIP events on its launch stream; achieved = bytes the fused schedule must '
+                'kernel': 'mmx_born_iterate_split: one Born iteration = 4 pencil-pass kernels on the launch '
+                          'stream (its FP64 norm reduction runs on a second stream, beside the next iteration), '
+                          'timed with HIP events on the launch stream; achieved = bytes the fused schedule must '
                           'move per iteration / event time',
                 'bytes_per_voxel_iter': round(b_must, 2), 'gpu_ms_per_iter_events': round(gpu_per_iter * 1e3, 4),
                 'alg_bytes_per_voxel_iter': b_alg,
