@@ -177,12 +177,13 @@ int mmx_born_iterate(mmx_plan_t plan, const void* E, void* Eout, void* W, const 
  * that norm_stream waits for: a call may reuse a slot only once the reduction of the previous call with that slot
  * has run (alternate the slots and wait on norm_stream for call n before issuing call n + 2), since it overwrites
  * the slot's partials and records its event again. Eout, W and norms as for mmx_born_iterate; norms is written on
- * norm_stream.
+ * norm_stream, and copied from there to `norms_host` (2 nbatch doubles of pinned host memory; NULL: no copy).
  * Replaces Solution.__iter__ (solver.py:1064-1098), as mmx_born_iterate.
  */
 int mmx_born_iterate_split(mmx_plan_t plan, const void* E, void* Eout, void* W, const void* s, const void* chi,
                            int ncomp, int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride,
-                           int64_t chi_bstride, double* norms, int slot, void* stream, void* norm_stream);
+                           int64_t chi_bstride, double* norms, double* norms_host, int slot, void* stream,
+                           void* norm_stream);
 
 /*
  * One Born iteration for a magnetic or bi-(an)isotropic medium (MacroMax Solution.__iter__ with the curl-term

@@ -77,7 +77,7 @@ def _declare(L):
     L.mmx_born_iterate.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p]
     L.mmx_born_iterate_split.restype = c_int
     L.mmx_born_iterate_split.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p,
-                                         c_int, c_p, c_p]
+                                         c_p, c_int, c_p, c_p]
     L.mmx_batch_step.restype = c_int
     L.mmx_batch_step.argtypes = [c_p, c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p, c_p, c_p,
                                  ctypes.c_double, c_i64, c_p]

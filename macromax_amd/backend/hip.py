@@ -225,12 +225,13 @@ class Plan:
         return W
 
     def born_iterate(self, E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch=1, s_bstride=0, chi_bstride=0,
-                     s_box=None, norm_stream=None, slot=0):
+                     s_box=None, norm_stream=None, slot=0, norms_host=None):
         """Finish the iteration begun in W (``mmx_born_iterate``): Eout = E + d, norms, W = next prologue.
 
         With ``norm_stream`` (a torch stream of the same device other than the current one), the norms are reduced
-        on that stream once the current one has run the passes, from partial buffer ``slot`` (0 or 1) of the plan
-        (``mmx_born_iterate_split``): a slot may be used again once the reduction of its previous call has run."""
+        on that stream once the current one has run the passes, from partial buffer ``slot`` (0 or 1) of the plan,
+        and copied there to ``norms_host`` (a pinned float64 CPU tensor, optional) (``mmx_born_iterate_split``): a
+        slot may be used again once the reduction of its previous call has run."""
         self._born_checks(E, ((Eout, 'Eout'), (W, 'W')), s, chi, ncomp, chi_kind, coef, nbatch, s_bstride,
                           chi_bstride)
         if norms.dtype != torch.float64 or norms.numel() < 2 * nbatch or norms.device != self.device:
@@ -241,7 +242,13 @@ class Plan:
         if norm_stream is None:
             _native.check(self._lib.mmx_born_iterate(*args, _stream_ptr(self.device)), 'mmx_born_iterate')
         else:
-            _native.check(self._lib.mmx_born_iterate_split(*args, int(slot), _stream_ptr(self.device),
+            host = None
+            if norms_host is not None:
+                if (norms_host.dtype != torch.float64 or norms_host.numel() < 2 * nbatch
+                        or norms_host.device.type != 'cpu' or not norms_host.is_pinned()):
+                    raise ValueError('norms_host must be a pinned float64 CPU tensor of 2*nbatch elements')
+                host = norms_host.data_ptr()
+            _native.check(self._lib.mmx_born_iterate_split(*args, host, int(slot), _stream_ptr(self.device),
                                                            norm_stream.cuda_stream), 'mmx_born_iterate_split')
         return norms
 
@@ -701,10 +708,11 @@ class BackEndHip(BackEnd):
                                          s_box=s_box)
 
     def born_iterate(self, E, Eout, W, s, chi, chi_kind: int, coef, norms, nbatch: int = 1, s_bstride: int = 0,
-                     chi_bstride: int = 0, s_box=None, norm_stream=None, slot: int = 0):
+                     chi_bstride: int = 0, s_box=None, norm_stream=None, slot: int = 0, norms_host=None):
         ncomp = E.shape[-self.grid.ndim - 2]
         return self.__plan.born_iterate(E, Eout, W, s, chi, ncomp, chi_kind, coef, norms, nbatch, s_bstride,
-                                        chi_bstride, s_box=s_box, norm_stream=norm_stream, slot=slot)
+                                        chi_bstride, s_box=s_box, norm_stream=norm_stream, slot=slot,
+                                        norms_host=norms_host)
 
     def __str__(self) -> str:
         return f'BackEndHip(vectorial={self.vectorial}, {self.grid}, dtype={self.numpy_dtype}, device={self.__device})'

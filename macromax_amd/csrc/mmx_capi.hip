@@ -670,6 +670,7 @@ int ensure_slot(mmx_plan_s* p, int slot, size_t bytes) {
 struct NormRoute {
     hipStream_t norm_stream = nullptr;
     int slot = 0;
+    double* norms_host = nullptr;  // pinned host copy of the norms (norm_stream only), or none
 };
 
 int partials_for(mmx_plan_s* p, const NormRoute& r, size_t bytes, double** partials) {
@@ -684,7 +685,11 @@ int reduce_routed(mmx_plan_s* p, const NormRoute& r, const double* partials, int
     if (!r.norm_stream) return reduce_norms(partials, tiles, nbatch, norms, stream);
     MMX_HIP_CHECK(hipEventRecord(p->slot_done[r.slot], stream));
     MMX_HIP_CHECK(hipStreamWaitEvent(r.norm_stream, p->slot_done[r.slot], 0));
-    return reduce_norms(partials, tiles, nbatch, norms, r.norm_stream);
+    const int rc = reduce_norms(partials, tiles, nbatch, norms, r.norm_stream);
+    if (rc || !r.norms_host) return rc;
+    MMX_HIP_CHECK(hipMemcpyAsync(r.norms_host, norms, (size_t)(2 * nbatch) * sizeof(double), hipMemcpyDeviceToHost,
+                                 r.norm_stream));
+    return MMX_OK;
 }
 
 template <typename T>
@@ -1452,7 +1457,8 @@ int mmx_born_iterate(mmx_plan_t p, const void* E, void* Eout, void* W, const voi
 
 int mmx_born_iterate_split(mmx_plan_t p, const void* E, void* Eout, void* W, const void* s, const void* chi,
                            int ncomp, int chi_kind, const void* coef, int64_t nbatch, int64_t s_bstride,
-                           int64_t chi_bstride, double* norms, int slot, void* stream, void* norm_stream) {
+                           int64_t chi_bstride, double* norms, double* norms_host, int slot, void* stream,
+                           void* norm_stream) {
     g_last_error.clear();
     if (!p || !E || !Eout || !W || !s || !chi || !coef || !norms || !norm_stream) return fail(MMX_EINVAL, "null argument");
     if (E == Eout || E == W || Eout == W) return fail(MMX_EINVAL, "E, Eout and W must be three different buffers");
@@ -1462,6 +1468,7 @@ int mmx_born_iterate_split(mmx_plan_t p, const void* E, void* Eout, void* W, con
     NormRoute route;
     route.norm_stream = reinterpret_cast<hipStream_t>(norm_stream);
     route.slot = slot;
+    route.norms_host = norms_host;
     return p->dtype == MMX_C64
                ? iterate_impl<float>(p, E, Eout, W, s, chi, ncomp, chi_kind, coef, nbatch, s_bstride, chi_bstride, norms,
                                      st, nullptr, route)

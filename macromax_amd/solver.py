@@ -54,6 +54,13 @@ class Solution:
     #: one by one; the results are identical). With the fused passes (8 launches per 2D iteration) and the next
     #: iteration launched ahead, direct launches measured faster (512^2: 0.140 against 0.149 ms per step)
     magnetic_graphs = False
+    #: problems with at least this much work per iteration, field values (voxels x components) x (non-trivial axes
+    #: + 1), reduce the norms and copy them to the host on a second stream beside the next iteration
+    #: (``mmx_born_iterate_split``): e.g. 1024^2 scalar and 48^3 vectorial grids and larger. Smaller ones, whose
+    #: host launches set the pace (25-30 us per iteration), keep both on the launch stream, which takes fewer host
+    #: calls (measured on MI355X: 512^2 scalar 29 against 36 us per iteration, 1024^2 43 against 38, 64^3
+    #: vectorial 50 against 44, 256^3 vectorial 824 against 806)
+    split_norms_min_work = 1 << 20
 
     def __init__(self, grid, vectorial=None, wavenumber=1.0, angular_frequency=None, vacuum_wavelength=None,
                  current_density=None, source_distribution=None, epsilon=None, xi=0.0, zeta=0.0, mu=1.0,
@@ -128,14 +135,12 @@ class Solution:
         self.__work_valid = False
         self.__norms_dev = [torch.zeros(2, dtype=torch.float64, device=BE.device) for _ in range(2)]
         self.__norms_host = [torch.zeros(2, dtype=torch.float64, pin_memory=True) for _ in range(2)]
-        # slot k's events: iteration done on the launch stream, norms copied to the host
-        self.__done_events = [torch.cuda.Event() for _ in range(2)]
+        # slot k's event: the iteration's norms are in host slot k
         self.__copy_events = [torch.cuda.Event() for _ in range(2)]
-        # On large grids (iterations of 0.2 ms and more) the norms' copies to the host run on a stream of their own
-        # (default priority), so that the next iteration's kernels do not queue behind them; on small grids, where
-        # the host's launches set the pace, the copy follows the iteration on its stream (fewer host calls).
+        # the second stream (default priority) of the norm reductions and copies, for problems large enough
         self.__copy_stream = None
-        if int(np.prod(self.grid.shape)) * (3 if self.vectorial else 1) >= 1 << 24:
+        values = int(np.prod(self.grid.shape)) * (3 if self.vectorial else 1)
+        if values * (sum(1 for n in self.grid.shape if n > 1) + 1) >= self.split_norms_min_work:
             self.__copy_stream = torch.cuda.Stream(device=BE.device)
             for t in self.__norms_dev:
                 t.record_stream(self.__copy_stream)  # not reused for other tensors before the copies end
@@ -645,32 +650,22 @@ class Solution:
             BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
                             self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box)
             return dst, k, self.__copy_norms(self.__norms_dev[k], k)
-        # the norm reduction and its copy run on the copy stream, behind the iteration's passes (partial slot k)
+        # the norm reduction and its copy to the host run on the copy stream, behind the iteration's passes
+        # (partial slot k)
         BE.born_iterate(self.__fields[src], self.__fields[dst], self.__work, self.__source_normalized, self.__chi,
                         self.__chi_kind, self.__coef, self.__norms_dev[k], s_box=self.__source_box,
-                        norm_stream=self.__copy_stream, slot=k)
-        with torch.cuda.stream(self.__copy_stream):
-            self.__norms_host[k].copy_(self.__norms_dev[k], non_blocking=True)
+                        norm_stream=self.__copy_stream, slot=k, norms_host=self.__norms_host[k])
         ev = self.__copy_events[k]
         ev.record(self.__copy_stream)
         return dst, k, ev
 
     def __copy_norms(self, norms_dev: torch.Tensor, k: int) -> torch.cuda.Event:
-        """Copy an iteration's two norms to pinned host slot k once the launching stream has computed them (on the
-        copy stream where there is one). Returns the event that marks the copy's completion. Slot k and its events
-        are used again two iterations later, after the host has waited for this event."""
-        launch = torch.cuda.current_stream(self.__BE.device)
+        """Copy an iteration's two norms to pinned host slot k behind the launch stream's work. Returns the event
+        that marks the copy's completion; slot k and its event are used again two iterations later, after the host
+        has waited for it."""
+        self.__norms_host[k].copy_(norms_dev, non_blocking=True)
         ev = self.__copy_events[k]
-        if self.__copy_stream is None:
-            self.__norms_host[k].copy_(norms_dev, non_blocking=True)
-            ev.record(launch)
-            return ev
-        done = self.__done_events[k]
-        done.record(launch)
-        self.__copy_stream.wait_event(done)
-        with torch.cuda.stream(self.__copy_stream):
-            self.__norms_host[k].copy_(norms_dev, non_blocking=True)
-        ev.record(self.__copy_stream)
+        ev.record(torch.cuda.current_stream(self.__BE.device))
         return ev
 
     def step(self) -> bool:
@@ -729,9 +724,6 @@ class Solution:
         runs eagerly, which also creates every lazily built constant outside the graph's memory pool."""
         if self.__mag_norms is None:
             self.__mag_norms = [torch.zeros(2, dtype=torch.float64, device=self.__BE.device) for _ in range(2)]
-            if self.__copy_stream is not None:
-                for t in self.__mag_norms:
-                    t.record_stream(self.__copy_stream)
         norms = self.__mag_norms[k]
         if not self.magnetic_graphs:
             self.__magnetic_compute(cur, nxt, norms)

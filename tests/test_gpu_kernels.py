@@ -360,13 +360,15 @@ def test_iterate_split_matches_iterate(shape, C, kind, dtype, device):
         W = torch.empty_like(Et)
         plan.born_prologue(bufs[0], W, st, ct, C, kind, coef)
         norms = [torch.full((2,), -1.0, dtype=torch.float64, device=device) for _ in range(4)]
+        hosts = [torch.full((2,), -2.0, dtype=torch.float64).pin_memory() for _ in range(4)]
         reduced = []
         for i in range(4):
             src, dst = bufs[i % 3], bufs[(i + 1) % 3]
             if split:
                 if i >= 2:
                     reduced[i - 2].synchronize()
-                plan.born_iterate(src, dst, W, st, ct, C, kind, coef, norms[i], norm_stream=norm_stream, slot=i % 2)
+                plan.born_iterate(src, dst, W, st, ct, C, kind, coef, norms[i], norm_stream=norm_stream, slot=i % 2,
+                                  norms_host=hosts[i] if i % 2 else None)
                 ev = torch.cuda.Event()
                 ev.record(norm_stream)
                 reduced.append(ev)
@@ -374,6 +376,9 @@ def test_iterate_split_matches_iterate(shape, C, kind, dtype, device):
                 plan.born_iterate(src, dst, W, st, ct, C, kind, coef, norms[i])
         torch.cuda.synchronize()
         runs[split] = ([b.cpu().numpy() for b in bufs], W.cpu().numpy(), [n.cpu().numpy() for n in norms])
+        if split:  # the host copies (odd calls) equal the device norms; no copy was asked for the even ones
+            for i in range(4):
+                assert np.array_equal(hosts[i].numpy(), runs[True][2][i] if i % 2 else np.full(2, -2.0))
     for b0, b1 in zip(runs[False][0], runs[True][0]):
         assert np.array_equal(b0, b1)
     assert np.array_equal(runs[False][1], runs[True][1])
