@@ -202,6 +202,14 @@ int mmx_born_iterate_split(mmx_plan_t plan, const void* E, void* Eout, void* W, 
 int mmx_magnetic_iterate(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* const* chi,
                          const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,
                          double* norms, void* stream);
+/*
+ * mmx_magnetic_iterate with its norm reduction on a second stream, from partial slot `slot`, and the norms copied to
+ * `norms_host` (pinned, or NULL), with the rules of mmx_born_iterate_split. Replaces the same reference code as
+ * mmx_magnetic_iterate.
+ */
+int mmx_magnetic_iterate_split(mmx_plan_t plan, const void* E, void* Eout, const void* s, const void* const* chi,
+                               const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,
+                               double* norms, double* norms_host, int slot, void* stream, void* norm_stream);
 
 /*
  * Batched solves: nbatch problems of one grid shape (many right-hand sides of one medium when chi_bstride = 0, or

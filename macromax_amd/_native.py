@@ -18,7 +18,7 @@ MMX_C128 = 1
 
 EXPORTED = ('mmx_version', 'mmx_last_error', 'mmx_register_fft_length', 'mmx_plan_create', 'mmx_plan_destroy', 'mmx_plan_set_param',
             'mmx_plan_get_param', 'mmx_plan_set_source_support', 'mmx_fft', 'mmx_born_step', 'mmx_born_step_noalloc',
-            'mmx_born_prologue', 'mmx_magnetic_iterate',
+            'mmx_born_prologue', 'mmx_magnetic_iterate', 'mmx_magnetic_iterate_split',
             'mmx_born_iterate', 'mmx_born_iterate_split',
             'mmx_batch_step', 'mmx_wall_clock', 'mmx_wall_clock_rate_khz', 'mmx_setup_sigma', 'mmx_setup_gain',
             'mmx_norm2', 'mmx_chi_mul',
@@ -71,6 +71,10 @@ def _declare(L):
     L.mmx_magnetic_iterate.restype = c_int
     L.mmx_magnetic_iterate.argtypes = [c_p, c_p, c_p, c_p, ctypes.POINTER(c_p), ctypes.POINTER(c_int),
                                        ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), c_p, c_p, c_p]
+    L.mmx_magnetic_iterate_split.restype = c_int
+    L.mmx_magnetic_iterate_split.argtypes = [c_p, c_p, c_p, c_p, ctypes.POINTER(c_p), ctypes.POINTER(c_int),
+                                             ctypes.POINTER(c_i64), ctypes.POINTER(ctypes.c_double), c_p, c_p, c_p,
+                                             c_int, c_p, c_p]
     L.mmx_born_prologue.restype = c_int
     L.mmx_born_prologue.argtypes = [c_p, c_p, c_p, c_p, c_p, c_int, c_int, c_p, c_i64, c_i64, c_i64, c_p]
     L.mmx_born_iterate.restype = c_int

@@ -178,11 +178,14 @@ class Plan:
         if s.numel() != (ncomp * self.V if s_bstride == 0 else ncomp * self.V * nbatch):
             raise ValueError('s size mismatch')
 
-    def magnetic_iterate(self, E, Eout, s, chis, alpha: complex, work, norms):
+    def magnetic_iterate(self, E, Eout, s, chis, alpha: complex, work, norms, norm_stream=None, slot=0,
+                         norms_host=None):
         """One magnetic Born iteration (``mmx_magnetic_iterate``): Eout = E + d, norms = (|d|^2, |Eout|^2).
 
         ``chis`` = (chi_EE, chi_HE, chi_EH, chi_HH), each a [1, 1, ...] scalar or [3, 3, ...] tensor field that is
-        either constant (one element per entry) or spans the whole grid (contiguous)."""
+        either constant (one element per entry) or spans the whole grid (contiguous). ``norm_stream``, ``slot``,
+        ``norms_host``: the norm reduction on a second stream, as for ``born_iterate``
+        (``mmx_magnetic_iterate_split``)."""
         for t, w in ((E, 'E'), (Eout, 'Eout'), (s, 's'), (work, 'work')):
             self._check(t, w)
         if E.numel() != 3 * self.V or Eout.numel() != E.numel() or s.numel() != E.numel():
@@ -207,11 +210,24 @@ class Plan:
         c_k = (ctypes.c_int * 4)(*kinds)
         c_s = (ctypes.c_int64 * 4)(*strides)
         c_a = (ctypes.c_double * 2)(float(alpha.real), float(alpha.imag))
+        args = (self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(), c_p, c_k, c_s, c_a, work.data_ptr(),
+                norms.data_ptr())
         with torch.cuda.device(self.device):
-            _native.check(self._lib.mmx_magnetic_iterate(self.handle, E.data_ptr(), Eout.data_ptr(), s.data_ptr(),
-                                                         c_p, c_k, c_s, c_a, work.data_ptr(), norms.data_ptr(),
-                                                         _stream_ptr(self.device)), 'mmx_magnetic_iterate')
+            if norm_stream is None:
+                _native.check(self._lib.mmx_magnetic_iterate(*args, _stream_ptr(self.device)), 'mmx_magnetic_iterate')
+            else:
+                host = None if norms_host is None else self._pinned_norms(norms_host, 1)
+                _native.check(self._lib.mmx_magnetic_iterate_split(*args, host, int(slot), _stream_ptr(self.device),
+                                                                   norm_stream.cuda_stream),
+                              'mmx_magnetic_iterate_split')
         return norms
+
+    @staticmethod
+    def _pinned_norms(norms_host, nbatch: int) -> int:
+        if (norms_host.dtype != torch.float64 or norms_host.numel() < 2 * nbatch or norms_host.device.type != 'cpu'
+                or not norms_host.is_pinned()):
+            raise ValueError('norms_host must be a pinned float64 CPU tensor of 2*nbatch elements')
+        return norms_host.data_ptr()
 
     def born_prologue(self, E, W, s, chi, ncomp, chi_kind, coef, nbatch=1, s_bstride=0, chi_bstride=0,
                       s_box=None):
@@ -242,12 +258,7 @@ class Plan:
         if norm_stream is None:
             _native.check(self._lib.mmx_born_iterate(*args, _stream_ptr(self.device)), 'mmx_born_iterate')
         else:
-            host = None
-            if norms_host is not None:
-                if (norms_host.dtype != torch.float64 or norms_host.numel() < 2 * nbatch
-                        or norms_host.device.type != 'cpu' or not norms_host.is_pinned()):
-                    raise ValueError('norms_host must be a pinned float64 CPU tensor of 2*nbatch elements')
-                host = norms_host.data_ptr()
+            host = None if norms_host is None else self._pinned_norms(norms_host, nbatch)
             _native.check(self._lib.mmx_born_iterate_split(*args, host, int(slot), _stream_ptr(self.device),
                                                            norm_stream.cuda_stream), 'mmx_born_iterate_split')
         return norms

@@ -1009,16 +1009,15 @@ bool mag_fused_ok(const mmx_plan_s* p) {
 }
 
 // The iteration with the fused passes (see mmx_magfused.hpp for the schedule).
-int magnetic_fused(mmx_plan_s* p, MagArgs<float>& a, double* norms, hipStream_t st) {
+int magnetic_fused(mmx_plan_s* p, MagArgs<float>& a, double* norms, hipStream_t st, const NormRoute& route) {
     const int64_t V = p->V;
     const bool three = p->naxes == 3;
     PassArgs<float> gf;
     TileCfg tf;
     int rc = mag_geometry(p, p->ax[p->naxes - 1], false, gf, tf);
     if (rc) return rc;
-    rc = ensure_scratch(p, (size_t)(2 * tf.nblocks) * sizeof(double));
+    rc = partials_for(p, route, (size_t)(2 * tf.nblocks) * sizeof(double), &a.partials);
     if (rc) return rc;
-    a.partials = reinterpret_cast<double*>(p->scratch);
     cx<float>* w1 = a.w1;
     cx<float>* w2 = a.w2;
     cx<float>* w3 = a.w3;
@@ -1043,12 +1042,13 @@ int magnetic_fused(mmx_plan_s* p, MagArgs<float>& a, double* norms, hipStream_t 
     if ((rc = mag_launch(p, 2, a, w2, w2, st))) return rc;       // i k x Y2^
     if (three && (rc = mag_middle_pass(p, w2, w2, 3, +1, st))) return rc;
     if ((rc = mag_launch(p, 5, a, w2, w1, st))) return rc;       // curl Y2; d, E', partials
-    return reduce_norms(a.partials, tf.nblocks, 1, norms, st);
+    return reduce_routed(p, route, a.partials, tf.nblocks, 1, norms, st);
 }
 
 template <typename T>
 int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const void* const* chi, const int* kinds,
-                  const int64_t* vstride, const double* alpha, void* work, double* norms, hipStream_t st) {
+                  const int64_t* vstride, const double* alpha, void* work, double* norms, hipStream_t st,
+                  const NormRoute& route = NormRoute()) {
     if (p->naxes == 0) return fail(MMX_EUNSUP, "grid without any axis longer than 1");
     MagArgs<T> a;
     memset(&a, 0, sizeof(a));
@@ -1081,12 +1081,13 @@ int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const
     a.alpha_r = T(alpha[0]);
     a.alpha_i = T(alpha[1]);
     if constexpr (sizeof(T) == 4) {
-        if (mag_fused_ok(p)) return magnetic_fused(p, a, norms, st);
+        if (mag_fused_ok(p)) return magnetic_fused(p, a, norms, st, route);
     }
     const int nblk = pointwise_blocks(V);
-    int rc = ensure_scratch(p, (size_t)(2 * nblk) * sizeof(double));
+    double* partials = nullptr;
+    int rc = partials_for(p, route, (size_t)(2 * nblk) * sizeof(double), &partials);
     if (rc) return rc;
-    a.partials = reinterpret_cast<double*>(p->scratch);
+    a.partials = partials;
     const dim3 grid(nblk), blk(256);
     auto check = [&]() -> int {
         MMX_HIP_CHECK(hipGetLastError());
@@ -1115,7 +1116,7 @@ int magnetic_impl(mmx_plan_s* p, const void* E, void* Eout, const void* s, const
     // d = Z2 + curl Y2, E' and the norms
     hipLaunchKernelGGL((magnetic_final_kernel<T>), grid, blk, 0, st, a);
     if ((rc = check())) return rc;
-    return reduce_norms(a.partials, nblk, 1, norms, st);
+    return reduce_routed(p, route, a.partials, nblk, 1, norms, st);
 }
 
 // Device address of `scratch` on the plan's device, looked up once per device (with that device current) and kept.
@@ -1430,6 +1431,25 @@ int mmx_magnetic_iterate(mmx_plan_t p, const void* E, void* Eout, const void* s,
     return p->dtype == MMX_C64
                ? magnetic_impl<float>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st)
                : magnetic_impl<double>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st);
+}
+
+int mmx_magnetic_iterate_split(mmx_plan_t p, const void* E, void* Eout, const void* s, const void* const* chi,
+                               const int* chi_kind, const int64_t* chi_vstride, const double* alpha, void* work,
+                               double* norms, double* norms_host, int slot, void* stream, void* norm_stream) {
+    g_last_error.clear();
+    if (!p || !E || !Eout || !s || !chi || !chi_kind || !chi_vstride || !alpha || !work || !norms || !norm_stream)
+        return fail(MMX_EINVAL, "null argument");
+    if (E == Eout || E == work || Eout == work) return fail(MMX_EINVAL, "E, Eout and work must be different buffers");
+    if (slot != 0 && slot != 1) return fail(MMX_EINVAL, "slot must be 0 or 1");
+    if (norm_stream == stream) return fail(MMX_EINVAL, "norm_stream must differ from stream");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    NormRoute route;
+    route.norm_stream = reinterpret_cast<hipStream_t>(norm_stream);
+    route.slot = slot;
+    route.norms_host = norms_host;
+    return p->dtype == MMX_C64
+               ? magnetic_impl<float>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st, route)
+               : magnetic_impl<double>(p, E, Eout, s, chi, chi_kind, chi_vstride, alpha, work, norms, st, route);
 }
 
 int mmx_born_prologue(mmx_plan_t p, const void* E, void* W, const void* s, const void* chi, int ncomp, int chi_kind,

@@ -164,15 +164,19 @@ class MagneticOperator:
             return t.reshape(n, n).contiguous()
         return _full(t, self.shape)
 
-    def iterate(self, E: torch.Tensor, Eout: torch.Tensor, norms: torch.Tensor):
+    def iterate(self, E: torch.Tensor, Eout: torch.Tensor, norms: torch.Tensor, norm_stream=None, slot: int = 0,
+                norms_host=None):
         """Eout = E + d, norms = (|d|^2, |Eout|^2) in FP64 with the fused kernel sequence (mmx_magnetic_iterate:
-        24 transformed components and five voxel-wise kernels per iteration)."""
+        24 transformed components and five voxel-wise kernels per iteration). With ``norm_stream`` the norms are
+        reduced on that stream from partial slot ``slot`` and copied to ``norms_host`` there
+        (mmx_magnetic_iterate_split)."""
         if self._ops is None or self._ops[0] != self.version:
             self._ops = (self.version, [self._kernel_operand(t) for t in (self.chiEE, self.chiHE, self.chiEH,
                                                                           self.chiHH)])
         if self._work is None:
             self._work = torch.empty((12, *self.shape), dtype=self.BE.hardware_dtype, device=self.BE.device)
-        self.plan.magnetic_iterate(E, Eout, self.s, self._ops[1], self.alpha, self._work, norms)
+        self.plan.magnetic_iterate(E, Eout, self.s, self._ops[1], self.alpha, self._work, norms,
+                                   norm_stream=norm_stream, slot=slot, norms_host=norms_host)
 
     def set_alpha(self, alpha_new: complex):
         """``solver.py:496-529`` for a new alpha (after a rejected update)."""

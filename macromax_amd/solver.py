@@ -54,6 +54,9 @@ class Solution:
     #: one by one; the results are identical). With the fused passes (8 launches per 2D iteration) and the next
     #: iteration launched ahead, direct launches measured faster (512^2: 0.140 against 0.149 ms per step)
     magnetic_graphs = False
+    #: magnetic media without graphs: reduce the norms and copy them to the host on a second stream beside the next
+    #: iteration (``mmx_magnetic_iterate_split``)
+    magnetic_split_norms = True
     #: problems with at least this much work per iteration, field values (voxels x components) x (non-trivial axes
     #: + 1), reduce the norms and copy them to the host on a second stream beside the next iteration
     #: (``mmx_born_iterate_split``): e.g. 1024^2 scalar and 48^3 vectorial grids and larger. Smaller ones, whose
@@ -150,6 +153,7 @@ class Solution:
         self.__mag_graph_state = None  # (operator version, plan scratch generation) the graphs were recorded for
         self.__mag_pool = None
         self.__mag_norms = None
+        self.__mag_stream = None
         self.E = initial_field
 
         def media_field(f):  # mu, xi, zeta as complex128 [1|3, 1|3, *grid-broadcastable] device tensors
@@ -722,9 +726,7 @@ class Solution:
         tensors, alpha and the plan's scratch address, so they are recorded again when the operator changes (new
         alpha or source) or the plan's scratch buffer has been reallocated. The first iteration after such a change
         runs eagerly, which also creates every lazily built constant outside the graph's memory pool."""
-        if self.__mag_norms is None:
-            self.__mag_norms = [torch.zeros(2, dtype=torch.float64, device=self.__BE.device) for _ in range(2)]
-        norms = self.__mag_norms[k]
+        norms = self.__magnetic_norms()[k]
         if not self.magnetic_graphs:
             self.__magnetic_compute(cur, nxt, norms)
             return
@@ -751,11 +753,29 @@ class Solution:
             self.__mag_graphs[key] = g
         g.replay()
 
+    def __magnetic_norms(self) -> list:
+        """The two device slots of the magnetic iteration's norms (allocated on first use)."""
+        if self.__mag_norms is None:
+            self.__mag_norms = [torch.zeros(2, dtype=torch.float64, device=self.__BE.device) for _ in range(2)]
+        return self.__mag_norms
+
     def __magnetic_launch(self, cur: int, nxt: int, k: int):
         """Enqueue one magnetic iteration (cur -> nxt) with its norms copied to pinned host slot k behind an event.
         Returns (nxt, k, event)."""
-        self.__magnetic_record(cur, nxt, k)
-        return nxt, k, self.__copy_norms(self.__mag_norms[k], k)
+        if self.magnetic_graphs or not self.magnetic_split_norms:
+            self.__magnetic_record(cur, nxt, k)
+            return nxt, k, self.__copy_norms(self.__mag_norms[k], k)
+        # the norm reduction and its copy to the host on a second stream, behind the iteration (partial slot k)
+        norms = self.__magnetic_norms()
+        if self.__mag_stream is None:
+            self.__mag_stream = torch.cuda.Stream(device=self.__BE.device)
+            for t in norms:
+                t.record_stream(self.__mag_stream)
+        self.__mag.iterate(self.__fields[cur], self.__fields[nxt], norms[k], norm_stream=self.__mag_stream, slot=k,
+                           norms_host=self.__norms_host[k])
+        ev = self.__copy_events[k]
+        ev.record(self.__mag_stream)
+        return nxt, k, ev
 
     def __magnetic_step(self) -> bool:
         """One Born iteration with the magnetic susceptibility operator (``solver.py:1064-1098``). As for other

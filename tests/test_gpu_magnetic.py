@@ -179,6 +179,42 @@ def test_magnetic_graph_replays_match_eager_launches(dtype):
     assert rejected >= 1
 
 
+@pytest.mark.parametrize('shape', [(64, 128), (64, 20, 64)], ids=str)
+def test_magnetic_split_norms_match_the_launch_stream_reduction(shape):
+    """The magnetic iteration with its norm reduction and host copy on a second stream
+    (Solution.magnetic_split_norms, mmx_magnetic_iterate_split; fused passes) gives the same fields, residues and
+    alpha as with both on the launch stream, over 12 iterations that include a rejected update."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print(f'magnetic split norms {shape}', flush=True)
+    rng = np.random.default_rng(4)
+    grid = mm.Grid(list(shape), 0.1e-6)
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2
+    mu = 1.0 + 0.3 * rng.random(shape)
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, 8] = 1.0
+    bound = LinearBound(grid, thickness=0.6e-6)
+
+    def make(split):
+        s = mm.Solution(grid, vacuum_wavelength=0.5e-6, current_density=J, epsilon=eps, mu=mu, bound=bound,
+                        dtype=np.complex64)
+        s.magnetic_split_norms = split
+        return s
+    a, b = make(False), make(True)
+    assert a.magnetic and b.magnetic
+    rejected = 0
+    for it in range(12):
+        if it == 7:
+            for s in (a, b):
+                s.field_array.mul_(50.0)
+        acc_a, acc_b = a.step(), b.step()
+        assert acc_a == acc_b, it
+        rejected += not acc_b
+        npt.assert_array_equal(b.field_array.cpu().numpy(), a.field_array.cpu().numpy(), err_msg=str(it))
+        assert b.residue == a.residue and b.alpha == a.alpha, it
+    assert rejected >= 1
+
+
 def test_magnetic_graphs_follow_a_reallocated_plan_scratch():
     """The plan's FP64 scratch buffer grows when a larger array is reduced on the same plan (MMX_PARAM_SCRATCH_GEN
     counts the allocations); graphs recorded before hold the old address, so the next magnetic step records them
