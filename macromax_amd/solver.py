@@ -148,6 +148,7 @@ class Solution:
             for t in self.__norms_dev:
                 t.record_stream(self.__copy_stream)  # not reused for other tensors before the copies end
         self.__field_fp64 = None
+        self.__refine_cache = None  # (FP64 field, its version, its FP64 residual source, relative residue)
         self.__BE64 = None
         self.__mag_graphs = {}  # (input buffer, output buffer) -> graph of one magnetic iteration
         self.__mag_graph_state = None  # (operator version, plan scratch generation) the graphs were recorded for
@@ -519,6 +520,7 @@ class Solution:
             t = t.unsqueeze(1)
         BE.assign(t, self.__fields[self.__cur])
         self.__field_fp64 = None
+        self.__refine_cache = None
         self.__previous_update_norm = math.inf
         self.__field_norm = None
         self.__invalidate_work()
@@ -645,6 +647,7 @@ class Solution:
         is void (the field, the source, the medium or alpha changed)."""
         self.__work_valid = False
         self.__pending = None
+        self.__refine_cache = None
 
     def __launch_iteration(self, src: int, dst: int, k: int):
         """Enqueue one fused iteration from field buffer src into dst, its norms into slot k (copied to pinned host
@@ -704,6 +707,7 @@ class Solution:
         if rel < 1:
             self.__cur = out
             self.__field_fp64 = None
+            self.__refine_cache = None
             self.__previous_update_norm = current
             self.__field_norm = math.sqrt(e2)
             return True
@@ -797,6 +801,7 @@ class Solution:
         if rel < 1:
             self.__cur = out
             self.__field_fp64 = None
+            self.__refine_cache = None
             self.__previous_update_norm = current
             self.__field_norm = math.sqrt(e2)
             return True
@@ -816,13 +821,31 @@ class Solution:
                                      device=self.__BE.device)
         return self.__BE64
 
-    def __fp64_residual_source(self, E: torch.Tensor):
+    def __fp64_chi_apply(self, X: torch.Tensor) -> torch.Tensor:
+        """chi_eff X in complex128 for X [C, 1, *grid], from chi as the kernels hold it (uniaxial media as
+        u_eff X + w_eff n (n . X), 3x3 media entry by entry), without forming a 3x3 field for a uniaxial chi."""
+        a, c = self.__coef_a, self.__coef_c
+        if self.__chi_kind == 4:
+            planes = self.__chi
+            real = torch.float32 if planes.dtype == torch.complex64 else torch.float64
+            shape = planes.shape[1:]
+            u = planes[0].to(torch.complex128) * a + c
+            w = planes[1].to(torch.complex128) * a
+            n = (planes[2].real.to(torch.float64), planes[2].imag.to(torch.float64),
+                 planes[3].view(real).reshape(-1)[:int(np.prod(shape))].reshape(shape).to(torch.float64))
+            ndot = w * (n[0] * X[0] + n[1] * X[1] + n[2] * X[2])
+            return torch.stack([u * X[i] + n[i] * ndot for i in range(3)])
+        chi = self.chi_effective.to(torch.complex128)
+        if chi.shape[0] == 1:
+            return chi * X
+        return torch.stack([chi[i, 0] * X[0] + chi[i, 1] * X[1] + chi[i, 2] * X[2] for i in range(3)])
+
+    def __fp64_residual_source(self, E: torch.Tensor) -> torch.Tensor:
         """s_c = s + chi E - G^-1 E in FP64, for the current alpha: the Born update of E is chi G s_c and the error
         of E solves the iteration with source s_c."""
         BE64 = self.__fp64_backend()
-        chi = self.chi_effective.to(torch.complex128)
         s = self.source_normalized.to(torch.complex128)
-        chiE = chi * E if chi.shape[0] == 1 else torch.einsum('ij...,jk...->ik...', chi, E)
+        chiE = self.__fp64_chi_apply(E)
         a = self.__alpha
         F = BE64.ft(E)
         ginv = (BE64.k2.to(torch.complex128) - a) * (1.0j / a.imag)  # 1/g, g = -i alpha_i / (k^2 - alpha)
@@ -831,9 +854,9 @@ class Solution:
             F = ginv * (F - PL) + PL * (-1.0j * a / a.imag)  # 1 / (i alpha_i / alpha) on the longitudinal part
         else:
             F = ginv * F
-        return s + chiE - BE64.ift(F), chi
+        return s + chiE - BE64.ift(F)
 
-    def __fp64_update_norm(self, s_c: torch.Tensor, chi: torch.Tensor) -> float:
+    def __fp64_update_norm(self, s_c: torch.Tensor) -> float:
         BE64 = self.__fp64_backend()
         a = self.__alpha
         F = BE64.ft(s_c)
@@ -843,9 +866,7 @@ class Solution:
             F = g * (F - PL) + PL * (1.0j * a.imag / a)
         else:
             F = g * F
-        G = BE64.ift(F)
-        d = chi * G if chi.shape[0] == 1 else torch.einsum('ij...,jk...->ik...', chi, G)
-        return float(torch.linalg.vector_norm(d))
+        return float(torch.linalg.vector_norm(self.__fp64_chi_apply(BE64.ift(F))))
 
     @property
     def field_array_fp64(self) -> torch.Tensor:
@@ -865,8 +886,13 @@ class Solution:
         self.__require_non_magnetic('refine')
         from .batch import BatchSolution
         E = self.field_array_fp64.clone()
-        s_c, chi = self.__fp64_residual_source(E)
-        res_before = self.__fp64_update_norm(s_c, chi) / float(torch.linalg.vector_norm(E))
+        cache = self.__refine_cache  # the residual source of the field the last sweep left, if still current
+        if cache is not None and cache[0] is self.__field_fp64 and cache[1] == cache[0]._version:
+            s_c, res_before = cache[2], cache[3]
+        else:
+            s_c = self.__fp64_residual_source(E)
+            res_before = self.__fp64_update_norm(s_c) / float(torch.linalg.vector_norm(E))
+        self.__refine_cache = None
         # source distribution S_c whose normalised, coefficient-scaled form is s_c
         S_c = s_c * (self.__alpha_base.imag * self.__beta / (1.0j * self.__coef_a))
         bt = BatchSolution.from_sources(self, sources=S_c[None]).solve(tol=tol, max_iter=max_iter,
@@ -877,8 +903,9 @@ class Solution:
         self.__invalidate_work()
         self.__field_norm = None
         self.__relative_residue = None
-        s_c, chi = self.__fp64_residual_source(E)
-        res_after = self.__fp64_update_norm(s_c, chi) / float(torch.linalg.vector_norm(E))
+        s_c = self.__fp64_residual_source(E)
+        res_after = self.__fp64_update_norm(s_c) / float(torch.linalg.vector_norm(E))
+        self.__refine_cache = (E, E._version, s_c, res_after)
         return {'fp64_residue_before': res_before, 'fp64_residue_after': res_after,
                 'correction_iterations': int(bt.iteration[0]), 'correction_residue': float(bt.residue[0])}
 
@@ -914,6 +941,7 @@ class Solution:
         """Take over the state a batched solve reached for this problem."""
         self.__fields[self.__cur].copy_(field)
         self.__field_fp64 = None
+        self.__refine_cache = None
         self.__iteration = int(iteration)
         self.__previous_update_norm = float(prev)
         self.__field_norm = float(field_norm) if field_norm > 0 else None

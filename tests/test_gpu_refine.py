@@ -34,7 +34,17 @@ def _case(kind):
     grid = mm.Grid(shape=list(shape), step=0.1e-6)
     sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, current_density=J,
                       bound=LinearBound(grid, thickness=0.6e-6), dtype=np.complex64)
-    chi = sol.chi_effective.cpu().numpy().astype(np.complex128)
+    # chi_eff as the FP64 refinement applies it (for this uniaxial medium u I + w n n^T from the stored u, w, n in
+    # FP64; test_fp64_chi_apply_of_a_uniaxial_medium compares that with the complex64 3x3 chi_eff), read off column by
+    # column from constant unit fields
+    C = 1 + 2 * sol.vectorial
+    dev = sol.field_array.device
+    cols = []
+    for j in range(C):
+        e = torch.zeros((C, 1, *shape), dtype=torch.complex128, device=dev)
+        e[j] = 1.0
+        cols.append(sol._Solution__fp64_chi_apply(e)[:, 0])
+    chi = torch.stack(cols, dim=1).cpu().numpy()
     s = sol.source_normalized.cpu().numpy().astype(np.complex128)
     k = ob.k_vectors(shape, np.asarray(grid.step) * sol.wavenumber)
     o = ob.solve_to(chi, s, sol.alpha, sol.beta, k, sol.vectorial, 1e-13)
@@ -51,7 +61,7 @@ def test_fp64_residual_source_vanishes_at_solution(kind):
     print(f'refine residual source {kind}', flush=True)
     sol, o = _case(kind)
     E = torch.as_tensor(o.E, device=sol.field_array.device)
-    s_c, chi = sol._Solution__fp64_residual_source(E)
+    s_c = sol._Solution__fp64_residual_source(E)
     s = sol.source_normalized.to(torch.complex128)
     assert float(torch.linalg.vector_norm(s_c) / torch.linalg.vector_norm(s)) < 1e-9
 
@@ -70,3 +80,24 @@ def test_refinement_reduces_error_below_complex64_floor(kind):
     assert e1 < 0.1 * e0 and e2 < 0.1 * e1
     assert e2 < 2e-6
     assert info2['fp64_residue_after'] < info1['fp64_residue_after'] < info1['fp64_residue_before']
+
+
+def test_fp64_chi_apply_of_a_uniaxial_medium():
+    """The refinement's FP64 chi_eff X of a uniaxial medium (chi stored as u I + w n n^T) equals the 3x3 chi_eff
+    applied entry by entry, also after a rejected update changed alpha and the coefficients."""
+    import bench
+    print('refine fp64 chi apply uniaxial', flush=True)
+    dev = torch.device('cuda:0')
+    sol, _ = bench.setup_solution(32, dev)
+    assert sol.chi_kind == 4
+    rng = np.random.default_rng(3)
+    X = torch.as_tensor(rng.standard_normal((3, 1, 32, 32, 32)) + 1j * rng.standard_normal((3, 1, 32, 32, 32)),
+                        device=dev)
+    for rejected in (False, True):
+        if rejected:
+            sol._set_alpha(complex(sol.alpha.real, 1.5 * sol.alpha.imag))
+        chi = sol.chi_effective.to(torch.complex128)
+        ref = torch.stack([sum(chi[i, j] * X[j] for j in range(3)) for i in range(3)])
+        got = sol._Solution__fp64_chi_apply(X)
+        assert got.shape == ref.shape
+        assert float(torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref)) < 1e-6
