@@ -185,6 +185,41 @@ def test_in_place_field_edit_voids_the_iteration_launched_ahead():
         assert err <= 1e-10, (it, err)
 
 
+@pytest.mark.parametrize('dtype', [np.complex64, np.complex128], ids=['c64', 'c128'])
+def test_split_norm_reduction_matches_the_launch_stream_reduction(dtype):
+    """On problems large enough, Solution reduces each iteration's norms on a second stream beside the next iteration
+    (mmx_born_iterate_split, two partial slots). Fields, residues and alpha equal those of the launch-stream
+    reduction bit for bit over 14 iterations, through a rejected update (field scaled in place), after which the
+    slots are reused while the voided iteration's reduction may still be queued."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    shape = (64, 64, 48)
+    rng = np.random.default_rng(6)
+    grid = mm.Grid(shape=list(shape), step=0.1e-6)
+    eps = (1.0 + 0.4 * rng.random(shape)) ** 2
+    J = np.zeros((3, *shape), dtype=np.complex64)
+    J[0, :, :, 10] = 1.0
+    sols = []
+    for split in (False, True):
+        sol = mm.Solution(grid, vacuum_wavelength=0.5e-6, epsilon=eps, current_density=J,
+                          bound=LinearBound(grid, thickness=0.6e-6), dtype=dtype)
+        sol._Solution__copy_stream = None if not split else sol._Solution__copy_stream
+        sols.append(sol)
+    plain, split = sols
+    assert split._Solution__copy_stream is not None  # 64 x 64 x 48 x 3 values x 4 > 2^20: the split path
+    rejected = 0
+    for it in range(14):
+        if it == 8:
+            for s in sols:
+                s.field_array.mul_(50.0)
+        acc_p, acc_s = plain.step(), split.step()
+        assert acc_p == acc_s, it
+        rejected += not acc_s
+        npt.assert_array_equal(split.field_array.cpu().numpy(), plain.field_array.cpu().numpy(), err_msg=str(it))
+        assert split.residue == plain.residue and split.alpha == plain.alpha, it
+    assert rejected >= 1
+
+
 def test_adaptive_stop_criterion():
     """macromax_amd.utils.Adaptive (utils/stop_criterion.py:7-52) as the solve callback: the solve stops at a reading
     below the target residue, within a few iterations of where the every-iteration rule stops."""
