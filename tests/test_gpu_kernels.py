@@ -386,6 +386,39 @@ def test_iterate_split_matches_iterate(shape, C, kind, dtype, device):
         assert np.array_equal(n0, n1) and np.all(n0 > 0)
 
 
+def test_iterate_split_batch_matches_iterate(device):
+    """mmx_born_iterate_split on a batch of three problems (own alpha and coefficients each, shared chi) gives the
+    fields and per-problem norms of mmx_born_iterate bit for bit; the host copy holds all 2 x 3 norms."""
+    import torch
+    shape, C, kind, nb = (64, 40, 32), 3, 4, 3
+    print(f'iterate split batch {shape} nbatch={nb}', flush=True)
+    E, s, chi, alphas, a, c = _born_case(shape, C, kind, nbatch=nb, seed=9)
+    plan = _plan(shape, [0.37] * 3, torch.complex64, device)
+    Et = torch.as_tensor(E, dtype=torch.complex64, device=device).contiguous()
+    st = torch.as_tensor(s, dtype=torch.complex64, device=device).contiguous()
+    ct = torch.as_tensor(_pack(chi[0], kind, np.complex64), device=device).contiguous()
+    coef = torch.tensor([[al.real, al.imag, a[b], c[b].real, c[b].imag, 0.0] for b, al in enumerate(alphas)],
+                        dtype=torch.float32, device=device)
+    norm_stream = torch.cuda.Stream(device=device)
+    out = {}
+    for split in (False, True):
+        W = torch.empty_like(Et)
+        Eo = torch.empty_like(Et)
+        norms = torch.full((2 * nb,), -1.0, dtype=torch.float64, device=device)
+        host = torch.full((2 * nb,), -2.0, dtype=torch.float64).pin_memory()
+        plan.born_prologue(Et, W, st, ct, C, kind, coef, nb, C * int(np.prod(shape)), 0)
+        if split:
+            plan.born_iterate(Et, Eo, W, st, ct, C, kind, coef, norms, nb, C * int(np.prod(shape)), 0,
+                              norm_stream=norm_stream, slot=1, norms_host=host)
+        else:
+            plan.born_iterate(Et, Eo, W, st, ct, C, kind, coef, norms, nb, C * int(np.prod(shape)), 0)
+        torch.cuda.synchronize()
+        out[split] = (Eo.cpu().numpy(), W.cpu().numpy(), norms.cpu().numpy(), host.numpy().copy())
+    assert np.array_equal(out[True][0], out[False][0]) and np.array_equal(out[True][1], out[False][1])
+    assert np.array_equal(out[True][2], out[False][2]) and np.all(out[False][2] > 0)
+    assert np.array_equal(out[True][3], out[False][2])
+
+
 def test_iterate_split_rejects_bad_arguments(device):
     """mmx_born_iterate_split refuses a slot other than 0 and 1 and a norm stream equal to the launch stream."""
     import torch
