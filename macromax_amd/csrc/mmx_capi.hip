@@ -918,10 +918,22 @@ inline bool mag_fused_length(const AxisPlan& ax) {
     return (n == 64 || n == 128 || n == 256 || n == 512 || n == 1024) && reg_supported(ax);
 }
 
+// Dynamic LDS of the two-group OP 1 mid pass: twiddles, an exchange buffer per group, the coupling buffer.
+constexpr size_t mag_split_lds(int N, int E, int P) {
+    return (size_t)(N + 2 * (size_t)P * padded_len(N, E) + 3 * (size_t)P * N) * sizeof(cx<float>);
+}
+
 template <int N, int OP>
 void launch_mag_mid(const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a, const cx<float>* src,
                     cx<float>* dst, hipStream_t st) {
     constexpr int E = reg_elems(N, 3, kGreen);
+    if constexpr (OP == 1) {
+        const size_t lds = mag_split_lds(N, E, t.P);
+        allow_lds<&mag_mid_split_kernel<float, N, E>>(lds);
+        hipLaunchKernelGGL((mag_mid_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
+                           g, a, src, dst);
+        return;
+    }
     constexpr int NC = OP == 1 ? 6 : 3;
     allow_lds<&mag_mid_kernel<float, N, E, NC, OP>>(t.lds);
     hipLaunchKernelGGL((mag_mid_kernel<float, N, E, NC, OP>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
@@ -953,12 +965,20 @@ void launch_mag_step(int step, const TileCfg& t, const PassArgs<float>& g, const
 
 // Geometry and tiles of a fused magnetic pass along `ax` (mid passes: the first axis, strided tiles of three
 // components as the Green pass; pw passes: the last axis, row tiles as the update pass).
-int mag_geometry(const mmx_plan_s* p, const AxisPlan& ax, bool mid, PassArgs<float>& g, TileCfg& t) {
+int mag_geometry(const mmx_plan_s* p, const AxisPlan& ax, bool mid, PassArgs<float>& g, TileCfg& t,
+                 bool split = false) {
     g = blank_args<float>();
     fill_geometry<float>(p, ax, g);
     const int kind = mid ? kGreen : kEpiPro;
-    const int rc = reg_tile_config(ax, 3, true, 1, mid ? p->p_green : p->p_rows, kind, t);
+    int rc = reg_tile_config(ax, 3, true, 1, mid ? p->p_green : p->p_rows, kind, t);
     if (rc) return rc;
+    if (split) {  // two thread groups per tile: at most 512 threads and a CU's LDS
+        const int TP = reg_threads_per_pencil(ax.N, 3, kind);
+        int P = t.P;
+        while (P > 1 && (2 * P * TP > 512 || mag_split_lds(ax.N, ax.N / TP, P) > p->lds_max)) P >>= 1;
+        if (P != t.P && (rc = reg_tile_config(ax, 3, true, 1, P, kind, t))) return rc;
+        if (2 * t.nth > 512) return fail(MMX_EINVAL, "internal: magnetic tile too wide");
+    }
     g.nouter = t.nouter;
     g.P = t.P;
     g.ntile_inner = (int)t.ntile_inner;
@@ -977,7 +997,7 @@ int mag_launch(const mmx_plan_s* p, int step, const MagArgs<float>& a, const cx<
     const AxisPlan& ax = mid ? p->ax[0] : p->ax[p->naxes - 1];
     PassArgs<float> g;
     TileCfg t;
-    int rc = mag_geometry(p, ax, mid, g, t);
+    int rc = mag_geometry(p, ax, mid, g, t, step == 1);
     if (rc) return rc;
     switch (ax.N) {
         case 64: launch_mag_step<64>(step, t, g, a, src, dst, st); break;

@@ -175,6 +175,106 @@ __global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> 
     mag_store<T, N, E, NC>(v, dst, tl, V);
 }
 
+// Mid pass OP 1 with its six components in two thread groups of P pencils each (blockDim = 2 P TP): group 0
+// transforms Z1 and writes B, group 1 transforms Y1 and writes curl B, each three components per thread (half the
+// serial work of the six-component kernel, whose passes at 0.26 M voxels are bound by one workgroup's chain). The
+// coupling goes through LDS after the forward FFTs: group 1 leaves i k x Y1^, group 0 forms
+// B^ = G (Z1^ + i k x Y1^) - E^ in place of it, group 1 takes B^ for i k x B^. Dynamic LDS: twiddles, one exchange
+// buffer per group, and the coupling buffer of 3 P N values ([component][pencil][point]).
+template <typename T, int N, int E>
+__global__ void __launch_bounds__(512) mag_mid_split_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src,
+                                                            cx<T>* dst) {
+    constexpr int TP = N / E;
+    constexpr int LTP = ilog2(TP);
+    extern __shared__ unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    const int P = g.P;
+    const int LP = 31 - __clz(P);
+    const int gsz = P * TP;
+    const int grp = (int)threadIdx.x >= gsz ? 1 : 0;
+    const int gtid = (int)threadIdx.x - grp * gsz;
+    const int plen = P * padded_len(N, E);
+    cx<T>* xbuf = tw + N + grp * plen;
+    cx<T>* xfer = tw + N + 2 * plen;
+    const bool rows = g.inner == 1;
+    int p, t, ps;
+    cx<T>* xb;
+    if (rows) { t = gtid & (TP - 1); p = gtid >> LTP; xb = xbuf + p * padded_len(N, E); ps = 1; }
+    else { p = gtid & (P - 1); t = gtid >> LP; xb = xbuf + p; ps = P; }
+    const MagTile tl = mag_tile<T, N, E>(g, p, t);
+    const int64_t V = a.V;
+    load_twiddles<T, N, E>(tw, g.tw, threadIdx.x, blockDim.x);
+    cx<T> v[3][E];
+    mag_load<T, N, E, 3>(v, src + grp * 3 * V, tl, V);
+    const int64_t col = tl.valid ? tl.col : 0;
+    const int64_t i1 = fdiv64(col, g.fn2), i2 = col - i1 * g.n2;
+    const T k1 = g.n1 > 1 ? g.kvec[g.k1ofs + i1] : T(0);
+    const T k2 = g.n2 > 1 ? g.kvec[g.k2ofs + i2] : T(0);
+    const bool dc_pencil = (i1 | i2) == 0;
+    T kpre[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) kpre[m] = g.kvec[t + m * TP];
+    cx<T> eh[3][E];  // E^ (group 0), requested before the forward FFT so that it arrives during it
+    if (grp == 0) mag_load<T, N, E, 3>(eh, a.w1, tl, V);
+    __syncthreads();
+    reg_fft<T, N, E, -1, 3>(v, t, tw, xb, ps);
+    cx<T>* xf = xfer + p * N + t;  // component c, point t + m TP of this pencil: xf[c P N + m TP]
+    if (grp == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const cx<T> y[3] = {v[0][m], v[1][m], v[2][m]};
+            cx<T> o[3];
+            cross_ik(kpre[m], k1, k2, y, o);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xf[c * P * N + m * TP] = o[c];
+        }
+    }
+    __syncthreads();
+    if (grp == 0) {
+        const T ar = a.alpha_r, inv_ai = T(1) / a.alpha_i;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const T k0 = kpre[m];
+            cx<T> f[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] = v[c][m] + xf[c * P * N + m * TP];
+            const T kk = k0 * k0 + k1 * k1 + k2 * k2;
+            const cx<T> gs = green_scalar(kk, ar, inv_ai, T(1));  // -i alpha_i / (k^2 - alpha)
+            const cx<T> gl = green_longitudinal(ar, inv_ai, T(1));  // i alpha_i / alpha
+            cx<T> b[3];
+            if (dc_pencil && t + m * TP == 0) {  // k = 0: the first nd components count as longitudinal
+#pragma unroll
+                for (int c = 0; c < 3; ++c) b[c] = (c < a.nd ? gl : gs) * f[c];
+            } else {
+                const cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
+                const cx<T> q = (gl - gs) * scal(proj, T(1) / kk);
+                b[0] = gs * f[0] + scal(q, k0);
+                b[1] = gs * f[1] + scal(q, k1);
+                b[2] = gs * f[2] + scal(q, k2);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                b[c] = b[c] - eh[c][m];
+                v[c][m] = b[c];
+                xf[c * P * N + m * TP] = b[c];
+            }
+        }
+    }
+    __syncthreads();
+    if (grp == 1) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const cx<T> b[3] = {xf[m * TP], xf[P * N + m * TP], xf[2 * P * N + m * TP]};
+            cx<T> cb[3];
+            cross_ik(kpre[m], k1, k2, b, cb);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) v[c][m] = cb[c];
+        }
+    }
+    reg_fft<T, N, E, +1, 3>(v, t, tw, xb, ps);
+    mag_store<T, N, E, 3>(v, dst + grp * 3 * V, tl, V);
+}
+
 // pw passes along the last (contiguous) grid axis: inverse FFT of NI components of `src` (the last inverse axis:
 // scaled by 1/V), voxel-wise products, then
 //   STAGE 1 (NI = 3: curl E): [Z1; Y1] = [chi_EE E + chi_HE curl E + s; chi_EH E + chi_HH curl E], forward FFT, to dst
