@@ -944,6 +944,13 @@ template <int N, int STAGE>
 void launch_mag_pw(const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a, const cx<float>* src,
                    cx<float>* dst, hipStream_t st) {
     constexpr int E = reg_elems(N, 3, kEpiPro);
+    if constexpr (STAGE == 1) {
+        const size_t lds = (size_t)(N + 2 * (size_t)t.P * padded_len(N, E)) * sizeof(cx<float>);
+        allow_lds<&mag_pw1_split_kernel<float, N, E>>(lds);
+        hipLaunchKernelGGL((mag_pw1_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
+                           g, a, src, dst);
+        return;
+    }
     allow_lds<&mag_pw_kernel<float, N, E, STAGE>>(t.lds);
     hipLaunchKernelGGL((mag_pw_kernel<float, N, E, STAGE>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
                        src, dst);
@@ -997,7 +1004,7 @@ int mag_launch(const mmx_plan_s* p, int step, const MagArgs<float>& a, const cx<
     const AxisPlan& ax = mid ? p->ax[0] : p->ax[p->naxes - 1];
     PassArgs<float> g;
     TileCfg t;
-    int rc = mag_geometry(p, ax, mid, g, t, step == 1);
+    int rc = mag_geometry(p, ax, mid, g, t, step == 1 || step == 3);
     if (rc) return rc;
     switch (ax.N) {
         case 64: launch_mag_step<64>(step, t, g, a, src, dst, st); break;

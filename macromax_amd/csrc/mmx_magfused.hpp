@@ -282,6 +282,62 @@ __global__ void __launch_bounds__(512) mag_mid_split_kernel(PassArgs<T> g, MagAr
 //   STAGE 2 (NI = 6: [B; curl B]): Z2 = chi_EE B + chi_HE curl B to w1 (real space), Y2 = chi_EH B + chi_HH curl B,
 //                             forward FFT, to dst (3 components);
 //   STAGE 3 (NI = 3: curl Y2): d = Z2 + curl Y2, E' = E + d to Eout, FP64 partials of |d|^2, |E'|^2 per tile.
+// pw pass STAGE 1 in two thread groups of P rows each (blockDim = 2 P TP): both take the inverse FFT of curl E^
+// (three components), then group 0 forms Z1 = chi_EE E + chi_HE curl E + s and group 1 Y1 = chi_EH E + chi_HH curl E,
+// and each transforms and writes its three components (dst + group 3 V). Dynamic LDS: twiddles and one exchange
+// buffer per group.
+template <typename T, int N, int E>
+__global__ void __launch_bounds__(512) mag_pw1_split_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src,
+                                                            cx<T>* dst) {
+    constexpr int TP = N / E;
+    constexpr int LTP = ilog2(TP);
+    extern __shared__ unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    const int gsz = g.P * TP;
+    const int grp = (int)threadIdx.x >= gsz ? 1 : 0;
+    const int gtid = (int)threadIdx.x - grp * gsz;
+    const int t = gtid & (TP - 1);
+    const int p = gtid >> LTP;
+    cx<T>* xb = tw + N + grp * g.P * padded_len(N, E) + p * padded_len(N, E);
+    const MagTile tl = mag_tile<T, N, E>(g, p, t);
+    const int64_t V = a.V;
+    const cx<T> zero{T(0), T(0)};
+    load_twiddles<T, N, E>(tw, g.tw, threadIdx.x, blockDim.x);
+    cx<T> x[3][E];
+    mag_load<T, N, E, 3>(x, src, tl, V);
+    // E, and s for group 0, requested before the inverse FFT so that they arrive during it
+    cx<T> e[3][E], sv[3][E];
+    mag_load<T, N, E, 3>(e, a.E, tl, V);
+    if (grp == 0) mag_load<T, N, E, 3>(sv, a.s, tl, V);
+    else {
+#pragma unroll
+        for (int m = 0; m < E; ++m)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sv[c][m] = zero;
+    }
+    __syncthreads();
+    reg_fft<T, N, E, +1, 3>(x, t, tw, xb, 1);
+    const T sc = g.scale;
+    const MagChi<T> ce = grp == 0 ? a.ee : a.eh;  // on E
+    const MagChi<T> cc = grp == 0 ? a.he : a.hh;  // on curl E
+    cx<T> y[3][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const cx<T> ev[3] = {e[0][m], e[1][m], e[2][m]};
+        cx<T> acc[3] = {sv[0][m], sv[1][m], sv[2][m]};
+        const cx<T> c[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
+        if (tl.valid) {
+            const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
+            mag_mul_add(ce, vox, V, ev, acc);
+            mag_mul_add(cc, vox, V, c, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) y[i][m] = acc[i];
+    }
+    reg_fft<T, N, E, -1, 3>(y, t, tw, xb, 1);
+    mag_store<T, N, E, 3>(y, dst + grp * 3 * V, tl, V);
+}
+
 template <typename T, int N, int E, int STAGE>
 __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
     constexpr int TP = N / E;
