@@ -922,6 +922,10 @@ inline bool mag_fused_length(const AxisPlan& ax) {
 constexpr size_t mag_split_lds(int N, int E, int P) {
     return (size_t)(N + 2 * (size_t)P * padded_len(N, E) + 3 * (size_t)P * N) * sizeof(cx<float>);
 }
+// ... and of the two-group pw pass STAGE 2, whose coupling buffer holds six components.
+constexpr size_t mag_pw2_lds(int N, int E, int P) {
+    return (size_t)(N + 2 * (size_t)P * padded_len(N, E) + 6 * (size_t)P * N) * sizeof(cx<float>);
+}
 
 template <int N, int OP>
 void launch_mag_mid(const TileCfg& t, const PassArgs<float>& g, const MagArgs<float>& a, const cx<float>* src,
@@ -948,6 +952,12 @@ void launch_mag_pw(const TileCfg& t, const PassArgs<float>& g, const MagArgs<flo
         const size_t lds = (size_t)(N + 2 * (size_t)t.P * padded_len(N, E)) * sizeof(cx<float>);
         allow_lds<&mag_pw1_split_kernel<float, N, E>>(lds);
         hipLaunchKernelGGL((mag_pw1_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
+                           g, a, src, dst);
+        return;
+    } else if constexpr (STAGE == 2) {
+        const size_t lds = mag_pw2_lds(N, E, t.P);
+        allow_lds<&mag_pw2_split_kernel<float, N, E>>(lds);
+        hipLaunchKernelGGL((mag_pw2_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
                            g, a, src, dst);
         return;
     }
@@ -982,7 +992,7 @@ int mag_geometry(const mmx_plan_s* p, const AxisPlan& ax, bool mid, PassArgs<flo
     if (split) {  // two thread groups per tile: at most 512 threads and a CU's LDS
         const int TP = reg_threads_per_pencil(ax.N, 3, kind);
         int P = t.P;
-        while (P > 1 && (2 * P * TP > 512 || mag_split_lds(ax.N, ax.N / TP, P) > p->lds_max)) P >>= 1;
+        while (P > 1 && (2 * P * TP > 512 || mag_pw2_lds(ax.N, ax.N / TP, P) > p->lds_max)) P >>= 1;
         if (P != t.P && (rc = reg_tile_config(ax, 3, true, 1, P, kind, t))) return rc;
         if (2 * t.nth > 512) return fail(MMX_EINVAL, "internal: magnetic tile too wide");
     }
@@ -1004,7 +1014,7 @@ int mag_launch(const mmx_plan_s* p, int step, const MagArgs<float>& a, const cx<
     const AxisPlan& ax = mid ? p->ax[0] : p->ax[p->naxes - 1];
     PassArgs<float> g;
     TileCfg t;
-    int rc = mag_geometry(p, ax, mid, g, t, step == 1 || step == 3);
+    int rc = mag_geometry(p, ax, mid, g, t, step == 1 || step == 3 || step == 4);
     if (rc) return rc;
     switch (ax.N) {
         case 64: launch_mag_step<64>(step, t, g, a, src, dst, st); break;

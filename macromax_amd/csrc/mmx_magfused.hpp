@@ -338,6 +338,65 @@ __global__ void __launch_bounds__(512) mag_pw1_split_kernel(PassArgs<T> g, MagAr
     mag_store<T, N, E, 3>(y, dst + grp * 3 * V, tl, V);
 }
 
+// pw pass STAGE 2 in two thread groups of P rows each: group 0 takes the inverse FFT of B^, group 1 that of
+// i k x B^; both leave their three components in LDS and read the other group's; group 0 writes
+// Z2 = chi_EE B + chi_HE curl B to w1, group 1 transforms Y2 = chi_EH B + chi_HH curl B and writes it to dst. Group 0
+// runs the same forward transform on its Z2 (result unused), so that both groups pass the same barriers. Dynamic
+// LDS: twiddles, one exchange buffer per group, and 6 P N values ([component][row][point]).
+template <typename T, int N, int E>
+__global__ void __launch_bounds__(512) mag_pw2_split_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src,
+                                                            cx<T>* dst) {
+    constexpr int TP = N / E;
+    constexpr int LTP = ilog2(TP);
+    extern __shared__ unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    const int P = g.P;
+    const int gsz = P * TP;
+    const int grp = (int)threadIdx.x >= gsz ? 1 : 0;
+    const int gtid = (int)threadIdx.x - grp * gsz;
+    const int t = gtid & (TP - 1);
+    const int p = gtid >> LTP;
+    cx<T>* xb = tw + N + grp * P * padded_len(N, E) + p * padded_len(N, E);
+    cx<T>* xf = tw + N + 2 * P * padded_len(N, E) + p * N + t;  // component c, point t + m TP: xf[c P N + m TP]
+    const MagTile tl = mag_tile<T, N, E>(g, p, t);
+    const int64_t V = a.V;
+    const cx<T> zero{T(0), T(0)};
+    load_twiddles<T, N, E>(tw, g.tw, threadIdx.x, blockDim.x);
+    cx<T> x[3][E];
+    mag_load<T, N, E, 3>(x, src + grp * 3 * V, tl, V);
+    __syncthreads();
+    reg_fft<T, N, E, +1, 3>(x, t, tw, xb, 1);
+    const T sc = g.scale;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) xf[(grp * 3 + c) * P * N + m * TP] = scal(x[c][m], sc);
+    __syncthreads();
+    const MagChi<T> cb_chi = grp == 0 ? a.ee : a.eh;  // on B
+    const MagChi<T> cc_chi = grp == 0 ? a.he : a.hh;  // on curl B
+    cx<T> y[3][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const cx<T> b[3] = {xf[m * TP], xf[P * N + m * TP], xf[2 * P * N + m * TP]};
+        const cx<T> cb[3] = {xf[3 * P * N + m * TP], xf[4 * P * N + m * TP], xf[5 * P * N + m * TP]};
+        cx<T> acc[3] = {zero, zero, zero};
+        if (tl.valid) {
+            const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
+            mag_mul_add(cb_chi, vox, V, b, acc);
+            mag_mul_add(cc_chi, vox, V, cb, acc);
+            if (grp == 0) {
+                cx<T>* zt = a.w1 + tl.tb;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) stg<true>(zt + i * V, tl.toff + m * tl.mstep, acc[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) y[i][m] = acc[i];
+    }
+    reg_fft<T, N, E, -1, 3>(y, t, tw, xb, 1);
+    if (grp == 1) mag_store<T, N, E, 3>(y, dst, tl, V);
+}
+
 template <typename T, int N, int E, int STAGE>
 __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
     constexpr int TP = N / E;
