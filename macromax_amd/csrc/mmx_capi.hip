@@ -936,12 +936,11 @@ void launch_mag_mid(const TileCfg& t, const PassArgs<float>& g, const MagArgs<fl
         allow_lds<&mag_mid_split_kernel<float, N, E>>(lds);
         hipLaunchKernelGGL((mag_mid_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
                            g, a, src, dst);
-        return;
+    } else {
+        allow_lds<&mag_mid_kernel<float, N, E, 3, OP>>(t.lds);
+        hipLaunchKernelGGL((mag_mid_kernel<float, N, E, 3, OP>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g,
+                           a, src, dst);
     }
-    constexpr int NC = OP == 1 ? 6 : 3;
-    allow_lds<&mag_mid_kernel<float, N, E, NC, OP>>(t.lds);
-    hipLaunchKernelGGL((mag_mid_kernel<float, N, E, NC, OP>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
-                       src, dst);
 }
 
 template <int N, int STAGE>
@@ -953,17 +952,16 @@ void launch_mag_pw(const TileCfg& t, const PassArgs<float>& g, const MagArgs<flo
         allow_lds<&mag_pw1_split_kernel<float, N, E>>(lds);
         hipLaunchKernelGGL((mag_pw1_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
                            g, a, src, dst);
-        return;
     } else if constexpr (STAGE == 2) {
         const size_t lds = mag_pw2_lds(N, E, t.P);
         allow_lds<&mag_pw2_split_kernel<float, N, E>>(lds);
         hipLaunchKernelGGL((mag_pw2_split_kernel<float, N, E>), dim3((unsigned)t.nblocks), dim3(2 * t.nth), lds, st,
                            g, a, src, dst);
-        return;
+    } else {
+        allow_lds<&mag_pw_kernel<float, N, E, STAGE>>(t.lds);
+        hipLaunchKernelGGL((mag_pw_kernel<float, N, E, STAGE>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g,
+                           a, src, dst);
     }
-    allow_lds<&mag_pw_kernel<float, N, E, STAGE>>(t.lds);
-    hipLaunchKernelGGL((mag_pw_kernel<float, N, E, STAGE>), dim3((unsigned)t.nblocks), dim3(t.nth), t.lds, st, g, a,
-                       src, dst);
 }
 
 // step 0..2: mid pass with k-space product OP = step; 3..5: pw pass STAGE = step - 2

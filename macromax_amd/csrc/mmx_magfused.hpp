@@ -87,11 +87,11 @@ __device__ __forceinline__ void mag_store(const cx<T> (&v)[NC][E], cx<T>* base, 
         for (int c = 0; c < NC; ++c) stg<true>(b + c * V, tl.toff + m * tl.mstep, v[c][m]);
 }
 
-// Mid passes along the first grid axis (g.axis_orig == 0): forward FFT of NC components of `src`, a k-space product,
-// inverse FFT, result to `dst` (unscaled).
-//   OP 0 (NC = 3): E^ to w1; i k x E^
-//   OP 1 (NC = 6): [Z1^; Y1^] -> B^ = G (Z1^ + i k x Y1^) - E^ (E^ from w1); [B^; i k x B^]
-//   OP 2 (NC = 3): i k x
+// Mid passes along the first grid axis (g.axis_orig == 0): forward FFT of NC = 3 components of `src`, a k-space
+// product, inverse FFT, result to `dst` (unscaled).
+//   OP 0: E^ to w1; i k x E^
+//   OP 2: i k x
+// (OP 1, [Z1^; Y1^] -> B^ = G (Z1^ + i k x Y1^) - E^ and [B^; i k x B^], runs in mag_mid_split_kernel.)
 template <typename T, int N, int E, int NC, int OP>
 __global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
     constexpr int TP = N / E;
@@ -121,55 +121,20 @@ __global__ void __launch_bounds__(512) mag_mid_kernel(PassArgs<T> g, MagArgs<T> 
     T kpre[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) kpre[m] = g.kvec[t + m * TP];
-    // OP 1: E^ is requested before the forward FFT, so that it arrives during it
-    cx<T> eh[OP == 1 ? 3 : 1][OP == 1 ? E : 1];
-    if constexpr (OP == 1) mag_load<T, N, E, 3>(eh, a.w1, tl, V);
+    static_assert(NC == 3 && (OP == 0 || OP == 2), "OP 1 runs in mag_mid_split_kernel");
     __syncthreads();
     reg_fft<T, N, E, -1, NC>(v, t, tw, xb, ps);
     const cx<T> zero{T(0), T(0)};
     if constexpr (OP == 0) {
         mag_store<T, N, E, NC>(v, a.w1, tl, V);
     }
-    const T ar = a.alpha_r, inv_ai = T(1) / a.alpha_i;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-        const T k0 = kpre[m];
-        if constexpr (OP == 0 || OP == 2) {
-            const cx<T> f[3] = {v[0][m], v[1][m], v[2][m]};
-            cx<T> o[3];
-            cross_ik(k0, k1, k2, f, o);
+        const cx<T> f[3] = {v[0][m], v[1][m], v[2][m]};
+        cx<T> o[3];
+        cross_ik(kpre[m], k1, k2, f, o);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) v[c][m] = o[c];
-        } else {
-            const cx<T> y[3] = {v[3][m], v[4][m], v[5][m]};
-            cx<T> f[3];
-            cross_ik(k0, k1, k2, y, f);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) f[c] = f[c] + v[c][m];
-            const T kk = k0 * k0 + k1 * k1 + k2 * k2;
-            const cx<T> gs = green_scalar(kk, ar, inv_ai, T(1));  // -i alpha_i / (k^2 - alpha)
-            const cx<T> gl = green_longitudinal(ar, inv_ai, T(1));  // i alpha_i / alpha
-            cx<T> b[3];
-            if (dc_pencil && t + m * TP == 0) {  // k = 0: the first nd components count as longitudinal
-#pragma unroll
-                for (int c = 0; c < 3; ++c) b[c] = (c < a.nd ? gl : gs) * f[c];
-            } else {
-                const cx<T> proj = scal(f[0], k0) + scal(f[1], k1) + scal(f[2], k2);
-                const cx<T> q = (gl - gs) * scal(proj, T(1) / kk);
-                b[0] = gs * f[0] + scal(q, k0);
-                b[1] = gs * f[1] + scal(q, k1);
-                b[2] = gs * f[2] + scal(q, k2);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) b[c] = b[c] - eh[c][m];
-            cx<T> cb[3];
-            cross_ik(k0, k1, k2, b, cb);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                v[c][m] = b[c];
-                v[3 + c][m] = cb[c];
-            }
-        }
+        for (int c = 0; c < 3; ++c) v[c][m] = o[c];
     }
     reg_fft<T, N, E, +1, NC>(v, t, tw, xb, ps);
     mag_store<T, N, E, NC>(v, dst, tl, V);
@@ -275,13 +240,14 @@ __global__ void __launch_bounds__(512) mag_mid_split_kernel(PassArgs<T> g, MagAr
     mag_store<T, N, E, 3>(v, dst + grp * 3 * V, tl, V);
 }
 
-// pw passes along the last (contiguous) grid axis: inverse FFT of NI components of `src` (the last inverse axis:
+// pw passes along the last (contiguous) grid axis: inverse FFT of the components of `src` (the last inverse axis:
 // scaled by 1/V), voxel-wise products, then
-//   STAGE 1 (NI = 3: curl E): [Z1; Y1] = [chi_EE E + chi_HE curl E + s; chi_EH E + chi_HH curl E], forward FFT, to dst
-//                             (6 components);
-//   STAGE 2 (NI = 6: [B; curl B]): Z2 = chi_EE B + chi_HE curl B to w1 (real space), Y2 = chi_EH B + chi_HH curl B,
-//                             forward FFT, to dst (3 components);
-//   STAGE 3 (NI = 3: curl Y2): d = Z2 + curl Y2, E' = E + d to Eout, FP64 partials of |d|^2, |E'|^2 per tile.
+//   STAGE 1 (curl E): [Z1; Y1] = [chi_EE E + chi_HE curl E + s; chi_EH E + chi_HH curl E], forward FFT, to dst
+//                     (6 components; mag_pw1_split_kernel);
+//   STAGE 2 ([B; curl B]): Z2 = chi_EE B + chi_HE curl B to w1 (real space), Y2 = chi_EH B + chi_HH curl B, forward
+//                     FFT, to dst (3 components; mag_pw2_split_kernel);
+//   STAGE 3 (curl Y2): d = Z2 + curl Y2, E' = E + d to Eout, FP64 partials of |d|^2, |E'|^2 per tile (mag_pw_kernel).
+//
 // pw pass STAGE 1 in two thread groups of P rows each (blockDim = 2 P TP): both take the inverse FFT of curl E^
 // (three components), then group 0 forms Z1 = chi_EE E + chi_HE curl E + s and group 1 Y1 = chi_EH E + chi_HH curl E,
 // and each transforms and writes its three components (dst + group 3 V). Dynamic LDS: twiddles and one exchange
@@ -399,9 +365,9 @@ __global__ void __launch_bounds__(512) mag_pw2_split_kernel(PassArgs<T> g, MagAr
 
 template <typename T, int N, int E, int STAGE>
 __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a, const cx<T>* src, cx<T>* dst) {
+    static_assert(STAGE == 3, "STAGE 1 and 2 run in the two-group pw kernels");
     constexpr int TP = N / E;
     constexpr int LTP = ilog2(TP);
-    constexpr int NI = STAGE == 2 ? 6 : 3;
     extern __shared__ unsigned char smem_raw[];
     __shared__ double red[16];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
@@ -412,66 +378,17 @@ __global__ void __launch_bounds__(512) mag_pw_kernel(PassArgs<T> g, MagArgs<T> a
     cx<T>* xb = xbuf + p * padded_len(N, E);
     const MagTile tl = mag_tile<T, N, E>(g, p, t);
     const int64_t V = a.V;
-    const cx<T> zero{T(0), T(0)};
     load_twiddles<T, N, E>(tw, g.tw, tid, blockDim.x);
-    cx<T> x[NI][E];
-    mag_load<T, N, E, NI>(x, src, tl, V);
-    // field operands of the voxel-wise products, requested before the inverse FFT so that they arrive during it:
-    // E and s (stage 1), Z2 and E (stage 3)
-    constexpr int NP = STAGE == 2 ? 1 : 3;
-    cx<T> f1[NP][STAGE == 2 ? 1 : E], f2[NP][STAGE == 2 ? 1 : E];
-    if constexpr (STAGE != 2) {
-        mag_load<T, N, E, 3>(f1, STAGE == 1 ? a.E : a.w1, tl, V);
-        mag_load<T, N, E, 3>(f2, STAGE == 1 ? a.s : a.E, tl, V);
-    }
+    cx<T> x[3][E];
+    mag_load<T, N, E, 3>(x, src, tl, V);
+    // Z2 and E, requested before the inverse FFT so that they arrive during it
+    cx<T> f1[3][E], f2[3][E];
+    mag_load<T, N, E, 3>(f1, a.w1, tl, V);
+    mag_load<T, N, E, 3>(f2, a.E, tl, V);
     __syncthreads();
-    reg_fft<T, N, E, +1, NI>(x, t, tw, xb, 1);
+    reg_fft<T, N, E, +1, 3>(x, t, tw, xb, 1);
     const T sc = g.scale;
-    if constexpr (STAGE == 1) {
-        cx<T> y[6][E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const cx<T> e[3] = {f1[0][m], f1[1][m], f1[2][m]};
-            cx<T> zz[3] = {f2[0][m], f2[1][m], f2[2][m]}, yy[3] = {zero, zero, zero};
-            const cx<T> c[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
-            if (tl.valid) {
-                const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
-                mag_mul_add(a.ee, vox, V, e, zz);
-                mag_mul_add(a.he, vox, V, c, zz);
-                mag_mul_add(a.eh, vox, V, e, yy);
-                mag_mul_add(a.hh, vox, V, c, yy);
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                y[i][m] = zz[i];
-                y[3 + i][m] = yy[i];
-            }
-        }
-        reg_fft<T, N, E, -1, 6>(y, t, tw, xb, 1);
-        mag_store<T, N, E, 6>(y, dst, tl, V);
-    } else if constexpr (STAGE == 2) {
-        cx<T> y[3][E];
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-            cx<T> zz[3] = {zero, zero, zero}, yy[3] = {zero, zero, zero};
-            const cx<T> b[3] = {scal(x[0][m], sc), scal(x[1][m], sc), scal(x[2][m], sc)};
-            const cx<T> cb[3] = {scal(x[3][m], sc), scal(x[4][m], sc), scal(x[5][m], sc)};
-            if (tl.valid) {
-                const int64_t vox = tl.tb + tl.toff / sizeof(cx<T>) + (int64_t)m * TP;
-                mag_mul_add(a.ee, vox, V, b, zz);
-                mag_mul_add(a.he, vox, V, cb, zz);
-                mag_mul_add(a.eh, vox, V, b, yy);
-                mag_mul_add(a.hh, vox, V, cb, yy);
-                cx<T>* zt = a.w1 + tl.tb;
-#pragma unroll
-                for (int i = 0; i < 3; ++i) stg<true>(zt + i * V, tl.toff + m * tl.mstep, zz[i]);
-            }
-#pragma unroll
-            for (int i = 0; i < 3; ++i) y[i][m] = yy[i];
-        }
-        reg_fft<T, N, E, -1, 3>(y, t, tw, xb, 1);
-        mag_store<T, N, E, 3>(y, dst, tl, V);
-    } else {
+    {
         double dd = 0.0, ee = 0.0;
         if (tl.valid) {
             cx<T>* Eo = a.Eout + tl.tb;
