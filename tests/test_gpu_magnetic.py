@@ -215,6 +215,41 @@ def test_magnetic_split_norms_match_the_launch_stream_reduction(shape):
     assert rejected >= 1
 
 
+def test_fused_magnetic_graph_replays_match_direct_launches():
+    """On a grid with fused magnetic passes (complex64, 64 x 128: the two-group passes with more than 64 KiB of LDS,
+    enabled per kernel on first launch), iterations replayed from HIP graphs give the fields, residues and alpha of
+    direct launches bit for bit, through a rejected update."""
+    import macromax_amd as mm
+    from macromax_amd.bound import LinearBound
+    print('fused magnetic graphs vs direct', flush=True)
+    shape = (64, 128)
+    rng = np.random.default_rng(8)
+    grid = mm.Grid(list(shape), 0.1e-6)
+    eps = (1.0 + 0.3 * rng.random(shape)) ** 2
+    mu = 1.0 + 0.3 * rng.random(shape)
+    J = np.zeros((3, *shape), dtype=np.complex128)
+    J[1, 8] = 1.0
+    bound = LinearBound(grid, thickness=0.6e-6)
+
+    def make(graphs):
+        s = mm.Solution(grid, vacuum_wavelength=0.5e-6, current_density=J, epsilon=eps, mu=mu, bound=bound,
+                        dtype=np.complex64)
+        s.magnetic_graphs = graphs
+        return s
+    direct, graphed = make(False), make(True)
+    rejected = 0
+    for it in range(12):
+        if it == 7:
+            for s in (direct, graphed):
+                s.field_array.mul_(50.0)
+        acc_d, acc_g = direct.step(), graphed.step()
+        assert acc_d == acc_g, it
+        rejected += not acc_g
+        npt.assert_array_equal(graphed.field_array.cpu().numpy(), direct.field_array.cpu().numpy(), err_msg=str(it))
+        assert graphed.residue == direct.residue and graphed.alpha == direct.alpha, it
+    assert rejected >= 1
+
+
 def test_magnetic_graphs_follow_a_reallocated_plan_scratch():
     """The plan's FP64 scratch buffer grows when a larger array is reduced on the same plan (MMX_PARAM_SCRATCH_GEN
     counts the allocations); graphs recorded before hold the old address, so the next magnetic step records them
